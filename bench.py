@@ -1,0 +1,195 @@
+"""Benchmark: converged HR cases/hour of the plain batched GPU solver.
+
+Workload (one GPU): the single-case GPU configuration of the reference's
+headline metric -- a symmetric diamond (chord 1, half-angle 5 deg) and a
+NACA0012 at h = 0.025, M in {0.9, 1.1, 2.0}, alpha = 2 deg, HLLC flux, MUSCL
+reconstruction, SSP-RK2, CFL 0.6 (0.45 in the transonic band), FP32 state
+with FP64 residual norms, run from freestream until the stationarity residual
+drops below 1e-6 (check every 5 steps, patience 1, cap 100k steps). Synthetic
+data: meshes from our own mesher, freestream initial states.
+
+A "step" of this benchmark is one batched solve of all six cases to
+convergence; ``value`` = converged cases per hour over the K timed solves.
+The roofline object is for the dominant kernel (the flux/update kernel),
+timed with HIP events on the solver's stream over back-to-back launches on
+the same states; its algorithmic bytes per launch are computed from the
+launch's work-items (see ``flux_bytes``).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def flux_bytes(items, cells):
+    """Compulsory HBM bytes of one stage-1 flux/update launch: per work-item
+    the stage state (16 B), the three limited face states (48 B) and the new
+    state (16 B); per group cell the geometry read once for all cases of the
+    group: 3 face words (12 B), 3 normals (24 B), 3 lengths (12 B), area and
+    sponge weight (8 B)."""
+    return items * 80 + cells * 56
+
+
+def recon_bytes(items, cells):
+    """Compulsory bytes of one reconstruction launch: state in (16 B) and three
+    face states out (48 B) per work-item; face words (12 B), face offsets
+    (24 B) and barycentre (8 B) per group cell."""
+    return items * 64 + cells * 44
+
+
+def build_cases(h):
+    from eulersr_amd import main as M
+    from eulersr_amd.meshing import build_mesh
+    meshes = {g: build_mesh(g, h) for g in ("diamond", "naca0012")}
+    specs, inits = [], []
+    for g, mesh in meshes.items():
+        for mach in (0.9, 1.1, 2.0):
+            cfg = dict(rho_inf=1.0, p_inf=1.0, gamma=1.4, Mach=mach, aoa=2.0, case=g, time_scheme="SRK2",
+                       flux="HLLC", reconstruction="MUSCL", CFL=0.6, tf=1.0e4)
+            W, inlet = M.initialize(mesh, cfg)
+            sp = M.case_spec(mesh, inlet, cfg)
+            sp.tag = dict(geometry=g, mach=mach)
+            specs.append(sp)
+            inits.append(W)
+    return meshes, specs, inits
+
+
+def solve_once(specs, inits, max_steps):
+    import torch
+    from eulersr_amd.solve import BatchSolver
+    bs = BatchSolver(specs, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=max_steps)
+    for i, W in enumerate(inits):
+        bs.set_state(i, W)
+    bs.prepare()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.run()
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    reps = [bs.report(i) for i in range(len(specs))]
+    return bs, t, reps
+
+
+def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
+    """numpy oracle (float32) SSP-RK2 steps on one HR mesh, 1 process."""
+    from oracle import fvm as F
+    om = F.omesh(mesh, np.float32)
+    W = np.asarray(spec.kw["value"], dtype=np.float32)[None, :].repeat(mesh.n_cells, 0)
+    kw = dict(spec.kw)
+    dt = F.get_dt(W, om, CFL=spec.cfl)
+    t0 = time.perf_counter()
+    W = F.time_step_RK2_SSP(W, om, dt, **kw)
+    one = time.perf_counter() - t0
+    n = max(1, int(target_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        W = F.time_step_RK2_SSP(W, om, dt, **kw)
+    t = time.perf_counter() - t0
+    cups = n * mesh.n_cells / t
+    cases_h = cups / (mesh.n_cells * steps_per_case) * 3600.0
+    return dict(value=cases_h, unit="cases/hour", cores=1, kind="port",
+                sample=(f"numpy oracle float32, {n} SSP-RK2 steps on diamond h=0.025 ({mesh.n_cells} cells) "
+                        f"in {t:.1f}s = {cups:.3g} cell-updates/s; cases/hour extrapolated with the GPU-measured "
+                        f"mean {steps_per_case:.0f} steps per converged case"),
+                cell_updates_per_s=cups)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--h", type=float, default=0.025)
+    ap.add_argument("--max-steps", type=int, default=100000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the cases of a batch are independent and would be sharded "
+              "by case across GPUs with no exchange; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        return 2
+    import torch
+    from eulersr_amd.device import require_gpu
+    require_gpu()
+    log(f"building meshes h={args.h}")
+    meshes, specs, inits = build_cases(args.h)
+    for g, m in meshes.items():
+        log(f"  {g}: {m.n_cells} cells")
+    # load code objects outside the timed region
+    solve_once(specs, inits, max_steps=20)
+    for w in range(args.warmup):
+        log(f"warmup solve {w + 1}/{args.warmup}")
+        solve_once(specs, inits, args.max_steps)
+    times, reps_all, last = [], [], None
+    for k in range(args.steps):
+        bs, t, reps = solve_once(specs, inits, args.max_steps)
+        times.append(t)
+        reps_all.append(reps)
+        log(f"timed solve {k + 1}/{args.steps}: {t:.3f}s steps={bs.steps} "
+            f"converged={sum(r['converged'] for r in reps)}/{len(reps)}")
+        if last is not None:
+            last.close()
+        last = bs
+    T = sum(times)
+    conv = sum(sum(r["converged"] for r in reps) for reps in reps_all)
+    cases_per_hour = conv / T * 3600.0
+    reps = reps_all[-1]
+    steps_case = [r["stopping_step"] if r["converged"] else r["steps_run"] for r in reps]
+    cells = [sp.mesh.n_cells for sp in specs]
+    cell_updates = sum(c * s for c, s in zip(cells, steps_case))
+    cups = cell_updates * args.steps / T
+
+    # dominant kernel: live HIP-event timing on the solver's stream
+    items, gcells = last.info()
+    ms_recon, ms_flux = last.time_kernels(args.kernel_reps)
+    fb, rb = flux_bytes(items, gcells), recon_bytes(items, gcells)
+    flux_gbs = fb / (ms_flux * 1e-3) / 1e9
+    recon_gbs = rb / (ms_recon * 1e-3) / 1e9 if ms_recon > 0 else 0.0
+    log(f"k_flux {ms_flux * 1e3:.1f} us/launch ({flux_gbs:.0f} GB/s), "
+        f"k_recon {ms_recon * 1e3:.1f} us/launch ({recon_gbs:.0f} GB/s), items {items}")
+
+    out = {
+        "metric": "converged HR cases/hour (plain batched solver, |dCD|<=1e-4 bar applies to warm starts)",
+        "value": cases_per_hour, "unit": "cases/hour", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": T / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain single-case GPU config: diamond (t/c tan 5deg) + NACA0012 at h=0.025, "
+                               "M in {0.9,1.1,2.0}, aoa 2deg, HLLC+MUSCL, SSP-RK2, CFL 0.6 (0.45 transonic), "
+                               "run to stationarity 1e-6 (check 5, patience 1, cap 100k steps)",
+                   "cases": len(specs), "cells": cells, "h": args.h},
+        "roofline": {"bound": "hbm", "achieved": flux_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": flux_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_flux<MUSCL,HLLC,stage1>", "bytes_per_launch": fb,
+                     "us_per_launch": ms_flux * 1e3},
+        "cell_updates_per_s": cups,
+        "steps_per_case": steps_case,
+        "converged": [bool(r["converged"]) for r in reps],
+        "cd": [r["cd"] for r in reps], "cl": [r["cl"] for r in reps],
+        "recon_kernel": {"us_per_launch": ms_recon * 1e3, "GB/s": recon_gbs, "bytes_per_launch": rb},
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (numpy oracle)")
+        mean_steps = float(np.mean([s for s, r in zip(steps_case, reps) if r["converged"]] or steps_case))
+        out["cpu_baseline"] = cpu_baseline(meshes["diamond"], specs[2], mean_steps)
+    last.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

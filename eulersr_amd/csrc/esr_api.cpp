@@ -1,0 +1,849 @@
+// Host orchestration and the C ABI (include/eulersr.h).
+//
+// An Engine holds the device state of a set of cases: cases sharing a mesh
+// and a kernel configuration form a group stored case-innermost; groups with
+// the same kernel configuration are launched together. The single-state
+// entry points (esr_residual, esr_step, ...) use a one-case Engine cached on
+// the mesh; the batched solver (esr_solver_*) owns its own Engine.
+//
+// Solve loop semantics follow euler/main.py:116-230: global dt computed once
+// from the initial state, N = int(tf/dt) + 1, a stationarity check every
+// K steps and at step N with patience, states frozen once stopped, a final
+// one-step probe residual.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <set>
+#include <tuple>
+
+#include "esr_launch.h"
+
+using namespace esr;
+
+namespace {
+
+Phys make_phys(double gamma, double M) {
+  Phys p;
+  p.g = (float)gamma;
+  p.gm1 = (float)(gamma - 1.0);
+  p.M2 = (float)(M * M);
+  p.gm1M2 = (float)((gamma - 1.0) * (M * M));
+  p.invM = (float)(1.0 / M);
+  p.invM2 = (float)(1.0 / (M * M));
+  p.two_g = (float)(2.0 * gamma);
+  p.c_ir1 = (float)((gamma + 1.0) / (2.0 * gamma));
+  p.c_ir2 = (float)((gamma - 1.0) / (2.0 * gamma));
+  p.one_m_g = (float)(1.0 - gamma);
+  p.k_exp = (float)(1.0 / (1.0 - gamma));
+  return p;
+}
+
+void validate_opts(const esr_opts* o) {
+  if (!o) throw Error(ESR_ERR_VALUE, "null options");
+  if (o->flux < ESR_FLUX_LF || o->flux > ESR_FLUX_ROE)
+    throw Error(ESR_ERR_VALUE, "Unknown flux type. Expected one of: 'lf', 'rusanov', 'tadmor', 'ausm', 'hll', 'hllc', 'roe'.");
+  if (o->recon != ESR_RECON_CONSTANT && o->recon != ESR_RECON_MUSCL)
+    throw Error(ESR_ERR_VALUE, "Unknown reconstruction scheme. Expected 'constant' or 'muscl'.");
+  if (!(o->gamma > 1.0)) throw Error(ESR_ERR_VALUE, "gamma must be > 1");
+  if (!(o->M > 0.0)) throw Error(ESR_ERR_VALUE, "M must be > 0");
+}
+
+// Sponge strength of helper.py:279-285 in float32: (|U_inf| + a_inf) / width
+// from the primitives of the inlet state.
+float sponge_strength(const esr_opts& o, float width) {
+  const Phys ph = make_phys(o.gamma, o.M);
+  const float r = o.value[0];
+  const float u = o.value[1] / r;
+  const float v = o.value[2] / r;
+  const float P = (o.value[3] - 0.5f * r * (u * u + v * v)) * ph.M2 * ph.gm1;
+  const float U = std::sqrt(u * u + v * v);
+  const float a = std::sqrt(ph.g * P / r) * ph.invM;
+  return 1.0f * (U + a) / width;
+}
+
+struct Stage {
+  int mode;
+  int ws;   // 0: W0, 1: W1, 2: W2
+  int out;  // 0: W0, 1: W1, 2: W2
+};
+
+std::vector<Stage> stages_of(int scheme) {
+  switch (scheme) {
+    case ESR_SCHEME_EE: return {{MODE_EE, 0, 1}};
+    case ESR_SCHEME_SRK2: return {{MODE_S1, 0, 1}, {MODE_SRK2_2, 1, 0}};
+    case ESR_SCHEME_RK2: return {{MODE_S1, 0, 1}, {MODE_RK2_2, 1, 0}};
+    case ESR_SCHEME_RK4:
+      return {{MODE_RK4_1, 0, 1}, {MODE_RK4_2, 1, 2}, {MODE_RK4_3, 2, 1}, {MODE_RK4_4, 1, 0}};
+    default: throw Error(ESR_ERR_KEY, "unknown time scheme (expected EE, RK2, RK4, SRK2, SSP_RK2)");
+  }
+}
+
+struct CaseSpec {
+  const esr_mesh* mesh;
+  esr_opts opts;
+};
+
+}  // namespace
+
+// Device state of a set of cases.
+struct Engine {
+  int32_t n_cases = 0;
+  int scheme = ESR_SCHEME_SRK2;
+  std::vector<GroupDev> groups;       // host copies
+  std::vector<const esr_mesh*> gmesh; // mesh of each group
+  std::vector<int32_t> case_group;    // global case -> group
+  std::vector<int32_t> case_slot;     // global case -> c within group
+  std::vector<CaseDev> cases_h;       // host copies of the case params
+  DevBuf groups_d, cases_d, loops_d;
+  struct Cls {
+    LaunchSet ls;
+    DevBuf blk;
+  };
+  std::vector<std::unique_ptr<Cls>> classes;
+  bool any_muscl = false, any_lf = false;
+  int64_t elems = 0, rec_elems = 0;
+  DevBuf W0, W1, W2, acc, Wref, rec;
+  int32_t rstride = 1;
+  DevBuf part_d, part_r, part_f, nranges, cfl_d, dt_d, forces_d, probe_d, count_d;
+  std::vector<std::unique_ptr<DevBuf>> custom_sponge;
+  int32_t* count_h = nullptr;  // pinned
+
+  ~Engine() {
+    if (count_h) (void)hipHostFree(count_h);
+  }
+
+  float4* buf(int which) {
+    return which == 0 ? W0.as<float4>() : which == 1 ? W1.as<float4>() : W2.as<float4>();
+  }
+
+  // Build groups from case specs. Cases with the same (mesh, flux, recon,
+  // entropy, gamma, M, sponge width) share a group; groups with the same
+  // (flux, recon, entropy) share a launch.
+  void build(const std::vector<CaseSpec>& specs, int scheme_) {
+    scheme = scheme_;
+    (void)stages_of(scheme);
+    n_cases = (int32_t)specs.size();
+    if (n_cases <= 0) throw Error(ESR_ERR_VALUE, "no cases");
+    using Key = std::tuple<const esr_mesh*, int, int, int, double, double, float>;
+    std::map<Key, std::vector<int32_t>> gm;
+    std::vector<Key> order;
+    for (int32_t i = 0; i < n_cases; ++i) {
+      const auto& s = specs[i];
+      if (!s.mesh) throw Error(ESR_ERR_VALUE, "case without a mesh");
+      validate_opts(&s.opts);
+      Key k{s.mesh, s.opts.flux, s.opts.recon, s.opts.entropy ? 1 : 0, s.opts.gamma, s.opts.M,
+            s.opts.sponge_width > 0 ? s.opts.sponge_width : -1.0f};
+      if (!gm.count(k)) order.push_back(k);
+      gm[k].push_back(i);
+    }
+    // global case order: group by group
+    std::vector<int32_t> user_of_global;
+    int32_t case0 = 0;
+    int64_t st = 0, rc = 0;
+    for (const Key& k : order) {
+      const auto& members = gm[k];
+      const esr_mesh* m = std::get<0>(k);
+      GroupDev g{};
+      g.m = m->dev;
+      g.C = (int32_t)members.size();
+      g.case0 = case0;
+      g.st_off = st;
+      g.ph = make_phys(std::get<4>(k), std::get<5>(k));
+      g.sponge = m->sponge;
+      const float width = std::get<6>(k);
+      if (width > 0) {
+        auto b = std::make_unique<DevBuf>();
+        upload(*b, sponge_weights(m, width));
+        g.sponge_g = b->as<float>();
+        custom_sponge.push_back(std::move(b));
+      } else {
+        g.sponge_g = m->dev.sponge_g;
+      }
+      st += (int64_t)m->n * g.C;
+      if (std::get<2>(k) == ESR_RECON_MUSCL) {
+        g.rec_off = rc;
+        rc += 3 * (int64_t)m->n * g.C;
+        any_muscl = true;
+      } else {
+        g.rec_off = 0;
+      }
+      if (std::get<1>(k) == ESR_FLUX_LF) any_lf = true;
+      for (int32_t c = 0; c < g.C; ++c) {
+        user_of_global.push_back(members[c]);
+        case_group.push_back((int32_t)groups.size());
+        case_slot.push_back(c);
+      }
+      case0 += g.C;
+      groups.push_back(g);
+      gmesh.push_back(m);
+    }
+    elems = st;
+    rec_elems = rc;
+    // launch classes
+    std::map<std::tuple<int, int, int>, std::vector<int32_t>> cls;
+    for (size_t gi = 0; gi < order.size(); ++gi) {
+      const Key& k = order[gi];
+      cls[{std::get<1>(k), std::get<2>(k), std::get<3>(k)}].push_back((int32_t)gi);
+    }
+    for (auto& kv : cls) {
+      auto c = std::make_unique<Cls>();
+      std::vector<int32_t> blk;
+      for (int32_t gi : kv.second) {
+        GroupDev& g = groups[gi];
+        const int64_t items = (int64_t)g.m.n * g.C;
+        g.blk0 = (int32_t)blk.size();
+        g.nblk = (int32_t)((items + BLOCK - 1) / BLOCK);
+        for (int32_t b = 0; b < g.nblk; ++b) blk.push_back(gi);
+      }
+      upload(c->blk, blk);
+      c->ls.blk_group = c->blk.as<int32_t>();
+      c->ls.nblocks = (int32_t)blk.size();
+      c->ls.flux = std::get<0>(kv.first);
+      c->ls.recon = std::get<1>(kv.first);
+      c->ls.entropy = std::get<2>(kv.first);
+      classes.push_back(std::move(c));
+    }
+    upload(groups_d, groups);
+    for (auto& c : classes) c->ls.groups = groups_d.as<GroupDev>();
+
+    // case params (user order -> global order)
+    cases_h.assign(n_cases, CaseDev{});
+    for (int32_t gc = 0; gc < n_cases; ++gc) {
+      const CaseSpec& s = specs[user_of_global[gc]];
+      CaseDev& cd = cases_h[gc];
+      cd.value = make_float4(s.opts.value[0], s.opts.value[1], s.opts.value[2], s.opts.value[3]);
+      const esr_mesh* m = s.mesh;
+      const float width = s.opts.sponge_width > 0 ? s.opts.sponge_width : 0.2f * (m->y_max - m->y_min);
+      cd.strength = s.opts.sponge_strength > 0 ? s.opts.sponge_strength : sponge_strength(s.opts, width);
+      cd.dt = cd.dt_half = cd.dt_sixth = 0.f;
+      cd.active = 1;
+    }
+    upload(cases_d, cases_h);
+    global_of_user.assign(n_cases, 0);
+    for (int32_t gc = 0; gc < n_cases; ++gc) global_of_user[user_of_global[gc]] = gc;
+
+    // state
+    W0.alloc(elems * sizeof(float4));
+    W1.alloc(elems * sizeof(float4));
+    Wref.alloc(elems * sizeof(float4));
+    if (scheme == ESR_SCHEME_RK4) {
+      W2.alloc(elems * sizeof(float4));
+      acc.alloc(elems * sizeof(float4));
+    }
+    rec.alloc((rec_elems > 0 ? rec_elems : 1) * sizeof(float4));
+    // reductions
+    rstride = 1;
+    std::vector<int32_t> nr(n_cases);
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      const int32_t r = reduce_ranges(groups[gi]);
+      rstride = std::max(rstride, r);
+      for (int32_t c = 0; c < groups[gi].C; ++c) nr[groups[gi].case0 + c] = r;
+    }
+    upload(nranges, nr);
+    part_d.alloc((size_t)n_cases * rstride * sizeof(double));
+    part_r.alloc((size_t)n_cases * rstride * sizeof(double));
+    part_f.alloc((size_t)n_cases * rstride * sizeof(float));
+    cfl_d.alloc(n_cases * sizeof(float));
+    dt_d.alloc(n_cases * sizeof(float));
+    forces_d.alloc(2 * (size_t)n_cases * sizeof(double));
+    probe_d.alloc(n_cases * sizeof(double));
+    count_d.alloc(sizeof(int32_t));
+    hip_check(hipHostMalloc((void**)&count_h, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+    *count_h = 0;
+  }
+
+  std::vector<int32_t> global_of_user;
+
+  CaseDev* cases() { return cases_d.as<CaseDev>(); }
+  const GroupDev* gdev(int32_t gi) const { return groups_d.as<GroupDev>() + gi; }
+
+  // one Runge-Kutta step of every active case
+  void step(hipStream_t st) {
+    for (const Stage& sg : stages_of(scheme)) {
+      const float4* Ws = buf(sg.ws);
+      for (auto& c : classes)
+        if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), Ws, rec.as<float4>(), st);
+      for (auto& c : classes) {
+        if (c->ls.flux == ESR_FLUX_LF) {
+          launch_zero_lf(n_cases, cases(), st);
+          launch_lf_cmax(c->ls, cases(), Ws, rec.as<float4>(), st);
+        }
+        launch_flux(c->ls, sg.mode, cases(), Ws, W0.as<float4>(), rec.as<float4>(), buf(sg.out), acc.as<float4>(), st);
+      }
+    }
+    if (scheme == ESR_SCHEME_EE)
+      for (auto& c : classes) launch_commit(c->ls, cases(), W1.as<float4>(), W0.as<float4>(), st);
+  }
+
+  // residual of W0 into W1 (one case engines)
+  void residual(hipStream_t st) {
+    for (auto& c : classes)
+      if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), W0.as<float4>(), rec.as<float4>(), st);
+    for (auto& c : classes) {
+      if (c->ls.flux == ESR_FLUX_LF) {
+        launch_zero_lf(n_cases, cases(), st);
+        launch_lf_cmax(c->ls, cases(), W0.as<float4>(), rec.as<float4>(), st);
+      }
+      launch_flux(c->ls, MODE_RESID, cases(), W0.as<float4>(), W0.as<float4>(), rec.as<float4>(), W1.as<float4>(),
+                  acc.as<float4>(), st);
+    }
+  }
+
+  void gather_in(int32_t user_case, const float* W, float4* slab, hipStream_t st) {
+    const int32_t gc = global_of_user.at(user_case);
+    const int32_t gi = case_group[gc];
+    launch_gather_in_dev(groups[gi].m.n, gdev(gi), case_slot[gc], gmesh[gi]->b_perm.as<int32_t>(),
+                         reinterpret_cast<const float4*>(W), slab, st);
+  }
+  void scatter_out(int32_t user_case, const float4* slab, float* W, hipStream_t st) {
+    const int32_t gc = global_of_user.at(user_case);
+    const int32_t gi = case_group[gc];
+    launch_scatter_out_dev(groups[gi].m.n, gdev(gi), case_slot[gc], gmesh[gi]->b_perm.as<int32_t>(), slab,
+                           reinterpret_cast<float4*>(W), st);
+  }
+
+  // per-case dt minimum of W0 (times cfl) -> device dt params and dt_d
+  void compute_dt(const std::vector<float>& cfl_user, hipStream_t st) {
+    std::vector<float> cfl(n_cases);
+    for (int32_t u = 0; u < n_cases; ++u) cfl[global_of_user[u]] = cfl_user[u];
+    hip_check(hipMemcpyAsync(cfl_d.p, cfl.data(), n_cases * sizeof(float), hipMemcpyHostToDevice, st), "cfl H2D");
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      launch_dt(groups[gi], gdev((int32_t)gi), cases(), W0.as<float4>(), part_f.as<float>(), rstride, 0, st);
+    launch_dt_final(n_cases, cases(), part_f.as<float>(), nranges.as<int32_t>(), rstride, cfl_d.as<float>(),
+                    dt_d.as<float>(), st);
+    hip_check(hipStreamSynchronize(st), "compute_dt sync");
+  }
+
+  // norms of (Wn - Wref) and Wref for cases flagged 'check'
+  void norms(const float4* Wn, float4* Wref_, int copy_ref, hipStream_t st) {
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      launch_norms(groups[gi], gdev((int32_t)gi), cases(), Wn, Wref_, part_d.as<double>(), part_r.as<double>(),
+                   rstride, copy_ref, st);
+  }
+  void forces(const float4* W, hipStream_t st) {
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      launch_forces(groups[gi], gdev((int32_t)gi), cases(), W, forces_d.as<double>(), st);
+  }
+};
+
+// single-state workspace attached to a mesh
+struct Workspace {
+  std::unique_ptr<Engine> eng;
+  esr_opts opts{};
+  int scheme = -1;
+  bool valid = false;
+};
+
+namespace esr {
+void free_workspace(Workspace* ws) { delete ws; }
+}  // namespace esr
+
+namespace {
+
+bool same_opts(const esr_opts& a, const esr_opts& b) {
+  return a.flux == b.flux && a.recon == b.recon && a.entropy == b.entropy && a.gamma == b.gamma && a.M == b.M &&
+         a.sponge_width == b.sponge_width && a.sponge_strength == b.sponge_strength &&
+         std::memcmp(a.value, b.value, sizeof(a.value)) == 0;
+}
+
+// One-case engine cached on the mesh, rebuilt when the options change.
+Engine& mesh_engine(esr_mesh* m, const esr_opts& o, int scheme, hipStream_t st) {
+  if (!m->ws) m->ws = new Workspace();
+  Workspace* ws = m->ws;
+  const bool need_rk4 = (scheme == ESR_SCHEME_RK4);
+  if (!ws->valid || !ws->eng || (need_rk4 && ws->scheme != ESR_SCHEME_RK4) ||
+      ws->eng->groups[0].ph.g != make_phys(o.gamma, o.M).g || o.flux != ws->opts.flux ||
+      o.recon != ws->opts.recon || o.entropy != ws->opts.entropy || o.gamma != ws->opts.gamma ||
+      o.M != ws->opts.M || o.sponge_width != ws->opts.sponge_width) {
+    hip_check(hipStreamSynchronize(st), "workspace sync");
+    ws->eng.reset(new Engine());
+    ws->eng->build({CaseSpec{m, o}}, need_rk4 ? ESR_SCHEME_RK4 : ESR_SCHEME_SRK2);
+    ws->opts = o;
+    ws->scheme = ws->eng->scheme;
+    ws->valid = true;
+  } else if (!same_opts(o, ws->opts)) {
+    // only the inlet value / sponge strength changed
+    CaseDev cd = ws->eng->cases_h[0];
+    cd.value = make_float4(o.value[0], o.value[1], o.value[2], o.value[3]);
+    const float width = o.sponge_width > 0 ? o.sponge_width : 0.2f * (m->y_max - m->y_min);
+    cd.strength = o.sponge_strength > 0 ? o.sponge_strength : sponge_strength(o, width);
+    ws->eng->cases_h[0] = cd;
+    launch_set_case(ws->eng->cases(), cd, st);
+    ws->opts = o;
+  }
+  Engine& e = *ws->eng;
+  e.scheme = scheme;
+  return e;
+}
+
+void set_dt(Engine& e, float dt, hipStream_t st) {
+  CaseDev cd = e.cases_h[0];
+  cd.dt = dt;
+  cd.dt_half = dt / 2.0f;
+  cd.dt_sixth = dt / 6.0f;
+  cd.active = 1;
+  cd.check = 0;
+  e.cases_h[0] = cd;
+  launch_set_case(e.cases(), cd, st);
+}
+
+template <class F>
+esr_status guard(F&& f) {
+  try {
+    f();
+    return ESR_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return ESR_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return ESR_ERR_DEVICE;
+  }
+}
+
+hipStream_t S(void* p) { return reinterpret_cast<hipStream_t>(p); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batched solver
+// ---------------------------------------------------------------------------
+
+struct esr_solver {
+  Engine eng;
+  esr_solve_opts so{};
+  std::vector<esr_case> cases;   // user order
+  std::vector<float> dt;         // user order
+  std::vector<int64_t> N;        // user order
+  std::vector<double> final_residual, cd, cl;
+  int64_t step = 0;
+  bool prepared = false;
+  bool finalized = false;
+};
+
+namespace {
+
+void solver_set_loops(esr_solver* s, hipStream_t st) {
+  Engine& e = s->eng;
+  std::vector<CaseLoop> L(e.n_cases);
+  for (int32_t u = 0; u < e.n_cases; ++u) {
+    const int32_t gc = e.global_of_user[u];
+    const esr_case& c = s->cases[u];
+    CaseLoop& l = L[gc];
+    std::memset(&l, 0, sizeof(l));
+    l.N = s->N[u];
+    l.ref_step = 0;
+    l.stopping_step = s->N[u];
+    l.last_rel = INFINITY;
+    l.threshold = s->so.threshold;
+    l.patience = std::max(1, s->so.patience);
+    l.q_ref = c.q_ref;
+    l.track_eng = c.track_engineering ? 1 : 0;
+    l.target_cd = c.target_cd;
+    l.target_cl = c.target_cl;
+    l.cd_tol = c.cd_tol_abs;
+    l.cl_tol = c.cl_tol_abs;
+    l.eng_patience = std::max(1, c.engineering_patience);
+    l.eng_stopping_step = s->N[u];
+  }
+  e.loops_d.alloc(L.size() * sizeof(CaseLoop));
+  hip_check(hipMemcpyAsync(e.loops_d.p, L.data(), L.size() * sizeof(CaseLoop), hipMemcpyHostToDevice, st),
+            "loops H2D");
+  hip_check(hipStreamSynchronize(st), "loops sync");
+}
+
+bool any_engineering(const esr_solver* s) {
+  for (const auto& c : s->cases)
+    if (c.track_engineering) return true;
+  return false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* esr_last_error(void) { return esr::last_error_cstr(); }
+
+int32_t esr_version(void) { return 100; }
+
+esr_status esr_mesh_create(const esr_mesh_desc* desc, esr_mesh** out) {
+  return guard([&] {
+    if (!out) throw Error(ESR_ERR_VALUE, "null output handle");
+    *out = build_mesh(desc);
+  });
+}
+
+esr_status esr_mesh_destroy(esr_mesh* mesh) {
+  return guard([&] { delete mesh; });
+}
+
+esr_status esr_mesh_permutation(const esr_mesh* mesh, int32_t* perm) {
+  return guard([&] {
+    if (!mesh || !perm) throw Error(ESR_ERR_VALUE, "null argument");
+    std::memcpy(perm, mesh->perm.data(), mesh->perm.size() * sizeof(int32_t));
+  });
+}
+
+esr_status esr_mesh_info(const esr_mesh* mesh, int32_t* n_cells, int32_t* n_wall_faces, int64_t* device_bytes) {
+  return guard([&] {
+    if (!mesh) throw Error(ESR_ERR_VALUE, "null mesh");
+    if (n_cells) *n_cells = mesh->n;
+    if (n_wall_faces) *n_wall_faces = mesh->dev.n_wall;
+    if (device_bytes) *device_bytes = mesh->device_bytes;
+  });
+}
+
+esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, int32_t* perm) {
+  return guard([&] {
+    if (n_cells <= 0 || !neighbors || !perm) throw Error(ESR_ERR_VALUE, "bad arguments");
+    for (int64_t t = 0; t < 3 * (int64_t)n_cells; ++t)
+      if (neighbors[t] < -1 || neighbors[t] >= n_cells) throw Error(ESR_ERR_VALUE, "neighbors out of range");
+    const std::vector<int32_t> p = rcm_order(n_cells, neighbors);
+    std::memcpy(perm, p.data(), p.size() * sizeof(int32_t));
+  });
+}
+
+esr_status esr_residual(esr_mesh* mesh, const esr_opts* opts, const float* W, float* R, void* stream) {
+  return guard([&] {
+    if (!mesh || !W || !R) throw Error(ESR_ERR_VALUE, "null argument");
+    validate_opts(opts);
+    hipStream_t st = S(stream);
+    Engine& e = mesh_engine(mesh, *opts, ESR_SCHEME_SRK2, st);
+    set_dt(e, 0.f, st);
+    e.gather_in(0, W, e.W0.as<float4>(), st);
+    e.residual(st);
+    e.scatter_out(0, e.W1.as<float4>(), R, st);
+  });
+}
+
+esr_status esr_step(esr_mesh* mesh, const esr_opts* opts, int32_t scheme, float dt, const float* W, float* W_out,
+                    void* stream) {
+  return guard([&] {
+    if (!mesh || !W || !W_out) throw Error(ESR_ERR_VALUE, "null argument");
+    validate_opts(opts);
+    (void)stages_of(scheme);
+    hipStream_t st = S(stream);
+    Engine& e = mesh_engine(mesh, *opts, scheme, st);
+    set_dt(e, dt, st);
+    e.gather_in(0, W, e.W0.as<float4>(), st);
+    e.step(st);
+    e.scatter_out(0, e.W0.as<float4>(), W_out, st);
+  });
+}
+
+esr_status esr_get_dt(esr_mesh* mesh, const float* W, float cfl, double gamma, double M, float* dt_out,
+                      void* stream) {
+  return guard([&] {
+    if (!mesh || !W || !dt_out) throw Error(ESR_ERR_VALUE, "null argument");
+    esr_opts o{};
+    o.flux = ESR_FLUX_HLLC;
+    o.recon = ESR_RECON_CONSTANT;
+    o.gamma = gamma;
+    o.M = M;
+    o.value[0] = 1.f;
+    o.value[1] = 1.f;
+    o.value[3] = 1.f;
+    o.sponge_width = -1.f;
+    o.sponge_strength = -1.f;
+    validate_opts(&o);
+    hipStream_t st = S(stream);
+    Engine& e = mesh_engine(mesh, o, ESR_SCHEME_SRK2, st);
+    set_dt(e, 0.f, st);
+    e.gather_in(0, W, e.W0.as<float4>(), st);
+    e.compute_dt({cfl}, st);
+    hip_check(hipMemcpy(dt_out, e.dt_d.p, sizeof(float), hipMemcpyDeviceToHost), "dt D2H");
+  });
+}
+
+esr_status esr_forces(esr_mesh* mesh, const float* W, double q_ref, double* cd, double* cl, void* stream) {
+  return guard([&] {
+    if (!mesh || !W || !cd || !cl) throw Error(ESR_ERR_VALUE, "null argument");
+    if (mesh->dev.n_wall == 0) throw Error(ESR_ERR_VALUE, "no wall face carries the force marker");
+    hipStream_t st = S(stream);
+    esr_opts o{};
+    o.flux = ESR_FLUX_HLLC;
+    o.recon = ESR_RECON_CONSTANT;
+    o.gamma = 1.4;
+    o.M = 1.0;
+    o.value[0] = 1.f;
+    o.value[1] = 1.f;
+    o.value[3] = 1.f;
+    o.sponge_width = -1.f;
+    o.sponge_strength = -1.f;
+    Engine& e = mesh_engine(mesh, mesh->ws && mesh->ws->valid ? mesh->ws->opts : o, ESR_SCHEME_SRK2, st);
+    e.gather_in(0, W, e.W0.as<float4>(), st);
+    launch_mark_check(1, e.cases(), nullptr, 0, 1, 1, st);
+    e.forces(e.W0.as<float4>(), st);
+    double f[2];
+    hip_check(hipMemcpyAsync(f, e.forces_d.p, sizeof(f), hipMemcpyDeviceToHost, st), "forces D2H");
+    hip_check(hipStreamSynchronize(st), "forces sync");
+    launch_set_active(1, e.cases(), nullptr, 1, st);
+    *cd = f[0] / q_ref;
+    *cl = f[1] / q_ref;
+  });
+}
+
+esr_status esr_norms(esr_mesh* mesh, const float* A, const float* B, double* diff_norm, double* ref_norm,
+                     void* stream) {
+  return guard([&] {
+    if (!mesh || !A || !B) throw Error(ESR_ERR_VALUE, "null argument");
+    hipStream_t st = S(stream);
+    esr_opts o = mesh->ws && mesh->ws->valid ? mesh->ws->opts : esr_opts{};
+    if (!(mesh->ws && mesh->ws->valid)) {
+      o.flux = ESR_FLUX_HLLC;
+      o.recon = ESR_RECON_CONSTANT;
+      o.gamma = 1.4;
+      o.M = 1.0;
+      o.value[0] = 1.f;
+      o.value[1] = 1.f;
+      o.value[3] = 1.f;
+      o.sponge_width = -1.f;
+      o.sponge_strength = -1.f;
+    }
+    Engine& e = mesh_engine(mesh, o, ESR_SCHEME_SRK2, st);
+    e.gather_in(0, A, e.W1.as<float4>(), st);
+    e.gather_in(0, B, e.Wref.as<float4>(), st);
+    launch_mark_check(1, e.cases(), nullptr, 0, 1, 1, st);
+    e.norms(e.W1.as<float4>(), e.Wref.as<float4>(), 0, st);
+    launch_probe_final(1, e.cases(), e.part_d.as<double>(), e.part_r.as<double>(), e.nranges.as<int32_t>(),
+                       e.rstride, e.probe_d.as<double>(), st);
+    std::vector<double> pd(e.rstride), pr(e.rstride);
+    int32_t nr = 0;
+    hip_check(hipMemcpyAsync(pd.data(), e.part_d.p, e.rstride * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(pr.data(), e.part_r.p, e.rstride * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(&nr, e.nranges.p, sizeof(int32_t), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "norms sync");
+    double d = 0.0, r = 0.0;
+    for (int32_t q = 0; q < nr; ++q) {
+      d += pd[q];
+      r += pr[q];
+    }
+    if (diff_norm) *diff_norm = std::sqrt(d);
+    if (ref_norm) *ref_norm = std::sqrt(r);
+  });
+}
+
+esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_solve_opts* so, esr_solver** out) {
+  return guard([&] {
+    if (!out || !cases || !so || n_cases <= 0) throw Error(ESR_ERR_VALUE, "bad solver arguments");
+    if (so->check_every < 1) throw Error(ESR_ERR_VALUE, "check_every must be >= 1");
+    if (so->local_dt) throw Error(ESR_ERR_VALUE, "local time stepping is not available in this build");
+    std::unique_ptr<esr_solver> s(new esr_solver());
+    s->so = *so;
+    s->cases.assign(cases, cases + n_cases);
+    std::vector<CaseSpec> specs;
+    for (int32_t i = 0; i < n_cases; ++i) specs.push_back(CaseSpec{cases[i].mesh, cases[i].opts});
+    s->eng.build(specs, so->scheme);
+    s->dt.assign(n_cases, 0.f);
+    s->N.assign(n_cases, 0);
+    s->final_residual.assign(n_cases, NAN);
+    s->cd.assign(n_cases, NAN);
+    s->cl.assign(n_cases, NAN);
+    *out = s.release();
+  });
+}
+
+esr_status esr_solver_destroy(esr_solver* s) {
+  return guard([&] { delete s; });
+}
+
+esr_status esr_solver_set_state(esr_solver* s, int32_t c, const float* W, void* stream) {
+  return guard([&] {
+    if (!s || !W || c < 0 || c >= s->eng.n_cases) throw Error(ESR_ERR_VALUE, "bad case index");
+    s->eng.gather_in(c, W, s->eng.W0.as<float4>(), S(stream));
+    s->prepared = false;
+  });
+}
+
+esr_status esr_solver_get_state(esr_solver* s, int32_t c, float* W, void* stream) {
+  return guard([&] {
+    if (!s || !W || c < 0 || c >= s->eng.n_cases) throw Error(ESR_ERR_VALUE, "bad case index");
+    s->eng.scatter_out(c, s->eng.W0.as<float4>(), W, S(stream));
+  });
+}
+
+esr_status esr_solver_prepare(esr_solver* s, void* stream) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    std::vector<float> cfl(e.n_cases);
+    for (int32_t u = 0; u < e.n_cases; ++u) cfl[u] = s->cases[u].cfl;
+    launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
+    e.compute_dt(cfl, st);
+    std::vector<float> dtg(e.n_cases);
+    hip_check(hipMemcpy(dtg.data(), e.dt_d.p, e.n_cases * sizeof(float), hipMemcpyDeviceToHost), "dt D2H");
+    for (int32_t u = 0; u < e.n_cases; ++u) {
+      const float dt = dtg[e.global_of_user[u]];
+      s->dt[u] = dt;
+      const float q = s->cases[u].tf / dt;  // float32 division, as tf / dt in the reference
+      if (!(q >= 0.f) || !std::isfinite(q)) throw Error(ESR_ERR_VALUE, "non-finite time step");
+      s->N[u] = (int64_t)q + 1;
+    }
+    solver_set_loops(s, st);
+    launch_copy(e.W0.as<float4>(), e.Wref.as<float4>(), e.elems, st);
+    s->step = 0;
+    s->prepared = true;
+    s->finalized = false;
+  });
+}
+
+esr_status esr_solver_run(esr_solver* s, int64_t* steps, void* stream) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    const int32_t K = s->so.check_every;
+    int64_t maxN = 0;
+    std::set<int64_t> Ns;
+    for (int64_t n : s->N) {
+      maxN = std::max(maxN, n);
+      Ns.insert(n);
+    }
+    int64_t limit = maxN;
+    if (s->so.max_steps > 0) limit = std::min(limit, s->so.max_steps);
+    const bool eng_on = any_engineering(s);
+    const int poll_every = std::max(1, 64 / K);
+    int checks = 0;
+    while (s->step < limit) {
+      s->step += 1;
+      e.step(st);
+      const int64_t k = s->step;
+      if (k % K == 0 || Ns.count(k)) {
+        launch_mark_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), k, K, 0, st);
+        e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st);
+        if (eng_on) e.forces(e.W0.as<float4>(), st);
+        launch_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), e.part_d.as<double>(), e.part_r.as<double>(),
+                     e.nranges.as<int32_t>(), e.rstride, eng_on ? e.forces_d.as<double>() : nullptr, k, K, st);
+        if (++checks % poll_every == 0) {
+          launch_count_active(e.n_cases, e.cases(), e.count_d.as<int32_t>(), st);
+          hip_check(hipMemcpyAsync(e.count_h, e.count_d.p, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count");
+          hip_check(hipStreamSynchronize(st), "poll sync");
+          if (*e.count_h == 0) break;
+        }
+      }
+    }
+    hip_check(hipStreamSynchronize(st), "run sync");
+    if (steps) *steps = s->step;
+  });
+}
+
+esr_status esr_solver_finalize(esr_solver* s, void* stream) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    // probe: one more step from the final state (main.py:228-230)
+    launch_copy(e.W0.as<float4>(), e.Wref.as<float4>(), e.elems, st);
+    launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
+    e.step(st);
+    launch_mark_check(e.n_cases, e.cases(), nullptr, 0, 1, 1, st);
+    e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 0, st);
+    launch_probe_final(e.n_cases, e.cases(), e.part_d.as<double>(), e.part_r.as<double>(), e.nranges.as<int32_t>(),
+                       e.rstride, e.probe_d.as<double>(), st);
+    launch_copy(e.Wref.as<float4>(), e.W0.as<float4>(), e.elems, st);
+    // forces of the final state
+    launch_mark_check(e.n_cases, e.cases(), nullptr, 0, 1, 1, st);
+    e.forces(e.W0.as<float4>(), st);
+    launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
+    std::vector<double> pr(e.n_cases), f(2 * (size_t)e.n_cases);
+    hip_check(hipMemcpyAsync(pr.data(), e.probe_d.p, pr.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipMemcpyAsync(f.data(), e.forces_d.p, f.size() * sizeof(double), hipMemcpyDeviceToHost, st), "D2H");
+    hip_check(hipStreamSynchronize(st), "finalize sync");
+    for (int32_t u = 0; u < e.n_cases; ++u) {
+      const int32_t gc = e.global_of_user[u];
+      s->final_residual[u] = pr[gc];
+      const double q = s->cases[u].q_ref;
+      s->cd[u] = q > 0 ? f[2 * (size_t)gc] / q : NAN;
+      s->cl[u] = q > 0 ? f[2 * (size_t)gc + 1] / q : NAN;
+    }
+    s->finalized = true;
+  });
+}
+
+esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out) {
+  return guard([&] {
+    if (!s || !out || c < 0 || c >= s->eng.n_cases) throw Error(ESR_ERR_VALUE, "bad case index");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    const Engine& e = s->eng;
+    CaseLoop L;
+    const int32_t gc = e.global_of_user[c];
+    hip_check(hipMemcpy(&L, e.loops_d.as<CaseLoop>() + gc, sizeof(L), hipMemcpyDeviceToHost), "loop D2H");
+    std::memset(out, 0, sizeof(*out));
+    out->dt = s->dt[c];
+    out->n_steps = s->N[c];
+    out->converged = L.stop;
+    out->stopping_step = L.stopping_step;
+    out->steps_run = L.done ? L.steps_run : s->step;
+    out->last_rel = L.last_rel;
+    out->final_residual = s->final_residual[c];
+    out->cd = s->cd[c];
+    out->cl = s->cl[c];
+    out->engineering_converged = L.eng_stop;
+    out->engineering_stopping_step = L.eng_stopping_step;
+    out->diverged = L.diverged;
+  });
+}
+
+esr_status esr_solver_info(const esr_solver* s, int64_t* work_items, int64_t* group_cells) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    int64_t items = 0, cells = 0;
+    for (const GroupDev& g : s->eng.groups) {
+      items += (int64_t)g.m.n * g.C;
+      cells += g.m.n;
+    }
+    if (work_items) *work_items = items;
+    if (group_cells) *group_cells = cells;
+  });
+}
+
+esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms_recon, double* ms_flux, void* stream) {
+  return guard([&] {
+    if (!s || reps <= 0) throw Error(ESR_ERR_VALUE, "bad arguments");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
+    hipEvent_t a, b, c;
+    hip_check(hipEventCreate(&a), "event");
+    hip_check(hipEventCreate(&b), "event");
+    hip_check(hipEventCreate(&c), "event");
+    // warm the code objects once
+    for (auto& cl : e.classes)
+      if (cl->ls.recon == ESR_RECON_MUSCL) launch_recon(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st);
+    for (auto& cl : e.classes)
+      launch_flux(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
+                  e.W1.as<float4>(), e.acc.as<float4>(), st);
+    hip_check(hipEventRecord(a, st), "record");
+    for (int32_t r = 0; r < reps; ++r)
+      for (auto& cl : e.classes)
+        if (cl->ls.recon == ESR_RECON_MUSCL)
+          launch_recon(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st);
+    hip_check(hipEventRecord(b, st), "record");
+    for (int32_t r = 0; r < reps; ++r)
+      for (auto& cl : e.classes)
+        launch_flux(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
+                    e.W1.as<float4>(), e.acc.as<float4>(), st);
+    hip_check(hipEventRecord(c, st), "record");
+    hip_check(hipEventSynchronize(c), "event sync");
+    float t1 = 0.f, t2 = 0.f;
+    hip_check(hipEventElapsedTime(&t1, a, b), "elapsed");
+    hip_check(hipEventElapsedTime(&t2, b, c), "elapsed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipEventDestroy(c);
+    if (s->prepared) launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
+    const double ncls = (double)e.classes.size();
+    if (ms_recon) *ms_recon = e.any_muscl ? (double)t1 / reps : 0.0;
+    if (ms_flux) *ms_flux = (double)t2 / reps / (ncls > 0 ? 1.0 : 1.0);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,1189 @@
+// gfx950 kernels of the explicit 2D unstructured finite-volume Euler step.
+//
+// One work-item per (cell, case). Cells are in reverse Cuthill-McKee order;
+// the state of a group of C cases that share a mesh is stored
+// case-innermost (element (cell, c) at cell*C + c), so neighbour gathers of
+// one wave hit contiguous memory when C >= 64 and geometry loads are
+// wave-uniform. Wave64 everywhere; no float atomics (the only atomic is an
+// integer max on non-negative float bits, which is order independent).
+//
+// Per stage of a Runge-Kutta step:
+//   k_recon  (MUSCL only): primitives, boundary ghosts, weighted LSQ
+//            gradient, Venkatakrishnan limiter and the three limited face
+//            states of each cell (helper.py:136-153, 233-256;
+//            euler_solver.py:36-122).
+//   k_flux   : per-side numerical flux of the three faces (the reference
+//            evaluates every interior face once from each side,
+//            euler_solver.py:107-109), face-length weighted sum, division by
+//            the area, sponge source and the stage update
+//            (euler_solver.py:141-143, 376-420, 602-667; helper.py:262-295).
+// Reductions (FP64 norms, FP64 CD/CL wall sums, dt minimum) use fixed-order
+// wave/LDS trees and per-range partials, so every run is bit-reproducible.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "esr_launch.h"
+
+namespace esr {
+
+// ---------------------------------------------------------------------------
+// state conversions (helper.py:25-46)
+// ---------------------------------------------------------------------------
+
+struct Prim {
+  float r, u, v, P;
+};
+
+// getPrimitive: P = (E - 0.5 rho (u^2+v^2)) * M^2 * (gamma-1)
+__device__ __forceinline__ Prim prim_of(const float4 W, float M2, float gm1) {
+  Prim q;
+  q.r = W.x;
+  q.u = W.y / W.x;
+  q.v = W.z / W.x;
+  q.P = (W.w - 0.5f * q.r * (q.u * q.u + q.v * q.v)) * M2 * gm1;
+  return q;
+}
+
+// getConserved: E = P / ((gamma-1) M^2) + 0.5 rho (u^2+v^2)
+__device__ __forceinline__ float4 cons_of(const Prim q, float gm1M2) {
+  return make_float4(q.r, q.r * q.u, q.r * q.v, q.P / gm1M2 + 0.5f * q.r * (q.u * q.u + q.v * q.v));
+}
+
+__device__ __forceinline__ Prim as_prim(const float4 p) { return Prim{p.x, p.y, p.z, p.w}; }
+
+// Boundary ghost state, helper.py:173-204 (walls and the subsonic inlet use
+// the default gamma = 1.4 and M = 1 of getPrimitive/getConserved).
+__device__ __forceinline__ float4 ghost_state(uint32_t kind, const float4 Wi, const float2 n, const float4 value,
+                                              const float4 insub, const float4* __restrict__ Ws, int64_t wrap_idx) {
+  if (kind == G_WALL) {
+    const Prim p = prim_of(Wi, 1.0f, 0.4f);
+    const float vn = (p.u - 0.0f) * n.x + (p.v - 0.0f) * n.y;
+    const float bx = (p.u - 0.0f) - 2.0f * vn * n.x;
+    const float by = (p.v - 0.0f) - 2.0f * vn * n.y;
+    return cons_of(Prim{p.r, bx + 0.0f, by + 0.0f, p.P}, 0.4f);
+  }
+  if (kind == G_INLET) return value;
+  if (kind == G_OUTLET) return Wi;
+  if (kind == G_SUBSONIC) {
+    const Prim p = prim_of(Wi, 1.0f, 0.4f);
+    const float r = insub.x, u = insub.y, v = insub.z;
+    return make_float4(r, r * u, r * v, p.P / 0.4f + 0.5f * r * (u * u + v * v));
+  }
+  return Ws[wrap_idx];
+}
+
+// ---------------------------------------------------------------------------
+// work-item mapping
+// ---------------------------------------------------------------------------
+
+struct Item {
+  const GroupDev* G;
+  int32_t cell;
+  int32_t c;
+  int64_t sidx;
+};
+
+__device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const int32_t* __restrict__ blk_group,
+                                         const CaseDev* __restrict__ cases, Item& it) {
+  const GroupDev* G = groups + blk_group[blockIdx.x];
+  const int32_t C = G->C;
+  const int64_t t = (int64_t)(blockIdx.x - G->blk0) * BLOCK + threadIdx.x;
+  const int32_t cell = (int32_t)(t / C);
+  const int32_t c = (int32_t)(t - (int64_t)cell * C);
+  if (cell >= G->m.n) return false;
+  if (!cases[G->case0 + c].active) return false;
+  it.G = G;
+  it.cell = cell;
+  it.c = c;
+  it.sidx = G->st_off + (int64_t)cell * C + c;
+  return true;
+}
+
+__device__ __forceinline__ int64_t sidx_of(const GroupDev* G, int32_t cell, int32_t c) {
+  return G->st_off + (int64_t)cell * G->C + c;
+}
+__device__ __forceinline__ int64_t ridx_of(const GroupDev* G, int k, int32_t cell, int32_t c) {
+  return G->rec_off + ((int64_t)k * G->m.n + cell) * G->C + c;
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction: LSQ gradient + Venkatakrishnan + MUSCL face states
+// ---------------------------------------------------------------------------
+
+// venkatakrishnan(a, b) with K = 0 (euler_solver.py:36-39)
+__device__ __forceinline__ float venkat(float a, float b) {
+  return (a * a + 2.0f * a * b) / (a * a + 2.0f * (b * b) + a * b + 1e-7f);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ groups,
+                                                 const int32_t* __restrict__ blk_group,
+                                                 const CaseDev* __restrict__ cases,
+                                                 const float4* __restrict__ Ws, float4* __restrict__ rec) {
+  Item it;
+  if (!map_item(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const Phys& ph = G->ph;
+  const int32_t n = m.n, cell = it.cell;
+  const CaseDev& cs = cases[G->case0 + it.c];
+
+  const float4 Wi = Ws[it.sidx];
+  const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
+  const float2 bi = m.bary[cell];
+
+  Prim pr[3];
+  float2 dl[3], dm[3];
+  bool bnd[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t f = m.fref[(int64_t)k * n + cell];
+    dm[k] = m.dxm[(int64_t)k * n + cell];
+    float4 WR;
+    if (fref_is_bnd(f)) {
+      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
+      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + cell] : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+      dl[k] = make_float2(2.0f * dm[k].x, 2.0f * dm[k].y);
+      bnd[k] = true;
+    } else {
+      const int32_t j = (int32_t)(f & FREF_IDX);
+      WR = Ws[sidx_of(G, j, it.c)];
+      if (f & FREF_PER) {
+        dl[k] = make_float2(2.0f * dm[k].x, 2.0f * dm[k].y);
+      } else {
+        const float2 bj = m.bary[j];
+        dl[k] = make_float2(bj.x - bi.x, bj.y - bi.y);
+      }
+      bnd[k] = false;
+    }
+    pr[k] = prim_of(WR, ph.M2, ph.gm1);
+  }
+
+  // weighted least squares, w = 1 / |dx|^2 (helper.py:144-152)
+  float wx[3], wy[3], wt[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float nr = sqrtf(dl[k].x * dl[k].x + dl[k].y * dl[k].y);
+    wt[k] = 1.0f / (nr * nr);
+    wx[k] = wt[k] * dl[k].x;
+    wy[k] = wt[k] * dl[k].y;
+  }
+  const float A00 = (wx[0] * dl[0].x + wx[1] * dl[1].x) + wx[2] * dl[2].x;
+  const float A01 = (wx[0] * dl[0].y + wx[1] * dl[1].y) + wx[2] * dl[2].y;
+  const float A10 = (wy[0] * dl[0].x + wy[1] * dl[1].x) + wy[2] * dl[2].x;
+  const float A11 = (wy[0] * dl[0].y + wy[1] * dl[1].y) + wy[2] * dl[2].y;
+  // LU with partial pivoting of the 2x2 system
+  const bool swp = fabsf(A10) > fabsf(A00);
+  const float p00 = swp ? A10 : A00, p01 = swp ? A11 : A01;
+  const float p10 = swp ? A00 : A10, p11 = swp ? A01 : A11;
+  const float l = p10 / p00;
+  const float u11 = p11 - l * p01;
+
+  const float qi[4] = {pi.r, pi.u, pi.v, pi.P};
+  float gx[4], gy[4], Wmin[4], Wmax[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    float q[3];
+    q[0] = (v == 0) ? pr[0].r : (v == 1) ? pr[0].u : (v == 2) ? pr[0].v : pr[0].P;
+    q[1] = (v == 0) ? pr[1].r : (v == 1) ? pr[1].u : (v == 2) ? pr[1].v : pr[1].P;
+    q[2] = (v == 0) ? pr[2].r : (v == 1) ? pr[2].u : (v == 2) ? pr[2].v : pr[2].P;
+    float bx = 0.f, by = 0.f;
+    float dW[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dW[k] = q[k] - qi[v];
+    bx = ((wt[0] * dW[0]) * dl[0].x + (wt[1] * dW[1]) * dl[1].x) + (wt[2] * dW[2]) * dl[2].x;
+    by = ((wt[0] * dW[0]) * dl[0].y + (wt[1] * dW[1]) * dl[1].y) + (wt[2] * dW[2]) * dl[2].y;
+    const float c0 = swp ? by : bx, c1 = swp ? bx : by;
+    const float y1 = c1 - l * c0;
+    const float x1 = y1 / u11;
+    const float x0 = (c0 - p01 * x1) / p00;
+    gx[v] = x0;
+    gy[v] = x1;
+    float mn = qi[v], mx = qi[v];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn = fminf(mn, q[k]);
+      mx = fmaxf(mx, q[k]);
+    }
+    Wmin[v] = mn;
+    Wmax[v] = mx;
+  }
+
+  // Venkatakrishnan limiter, min over the three faces (euler_solver.py:45-67)
+  float D[3][4];
+  float phi[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    float ph3[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float d = dm[k].x * gx[v] + dm[k].y * gy[v];
+      D[k][v] = d;
+      float p = 1.0f;
+      if (d > 1e-7f) p = venkat(Wmax[v] - qi[v], d);
+      if (d < -1e-7f) p = venkat(Wmin[v] - qi[v], d);
+      ph3[k] = p;
+    }
+    phi[v] = fminf(fminf(ph3[0], ph3[1]), ph3[2]);
+  }
+
+  // limited face states (primitives); no limiting slope on marker > 1 faces
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float4 o;
+    const float s0 = bnd[k] ? 0.0f : phi[0];
+    const float s1 = bnd[k] ? 0.0f : phi[1];
+    const float s2 = bnd[k] ? 0.0f : phi[2];
+    const float s3 = bnd[k] ? 0.0f : phi[3];
+    o.x = qi[0] + s0 * D[k][0];
+    o.y = qi[1] + s1 * D[k][1];
+    o.z = qi[2] + s2 * D[k][2];
+    o.w = qi[3] + s3 * D[k][3];
+    rec[ridx_of(G, k, cell, it.c)] = o;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// numerical fluxes (euler_solver.py:145-591); each returns the total flux
+// F = central - D of one side-face.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void central_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R,
+                                             float nx, float ny, float i2, float F[4]) {
+  // getFlux central part (euler_solver.py:382-417)
+  const float unL = L.u * nx + L.v * ny;
+  const float unR = R.u * nx + R.v * ny;
+  F[0] = 0.5f * (L.r * unL + R.r * unR);
+  F[1] = 0.5f * ((L.r * L.u * unL + i2 * L.P * nx) + (R.r * R.u * unR + i2 * R.P * nx));
+  F[2] = 0.5f * ((L.r * L.v * unL + i2 * L.P * ny) + (R.r * R.v * unR + i2 * R.P * ny));
+  F[3] = 0.5f * ((WL.w + i2 * L.P) * unL + (WR.w + i2 * R.P) * unR);
+}
+
+// physical flux in the HLLC/AUSM form: P n / M^2
+__device__ __forceinline__ void phys_flux(const float4 W, const Prim& q, float un, float nx, float ny, float invM2,
+                                          float F[4]) {
+  F[0] = q.r * un;
+  F[1] = q.r * q.u * un + q.P * nx * invM2;
+  F[2] = q.r * q.v * un + q.P * ny * invM2;
+  F[3] = (W.w + q.P * invM2) * un;
+}
+
+__device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float D[4]) {
+  const float unL = L.u * nx + L.v * ny;
+  const float unR = R.u * nx + R.v * ny;
+  const float tx = -ny, ty = nx;
+  const float utL = L.u * tx + L.v * ty;
+  const float utR = R.u * tx + R.v * ty;
+  const float aL = sqrtf(fabsf(ph.g * L.P / L.r)) * ph.invM;
+  const float aR = sqrtf(fabsf(ph.g * R.P / R.r)) * ph.invM;
+  float SL = fminf(unL - aL, unR - aR);
+  float SR = fmaxf(unL + aL, unR + aR);
+  SL = (SL == SR) ? SL - 1e-6f : SL;
+  SR = (SL == SR) ? SR + 1e-6f : SR;
+  float FL[4], FR[4];
+  phys_flux(WL, L, unL, nx, ny, ph.invM2, FL);
+  phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
+  float den = L.r * (SL - unL) - R.r * (SR - unR);
+  den = (den == 0.0f) ? 1e-12f : den;
+  const float Ss = (R.P - L.P + L.r * unL * (SL - unL) - R.r * unR * (SR - unR)) / den;
+  float F[4];
+  if (SL >= 0.0f) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) F[v] = FL[v];
+  } else {
+    const float PsL = L.P + L.r * (SL - unL) * (Ss - unL);
+    const float PsR = R.P + R.r * (SR - unR) * (Ss - unR);
+    const float Ps = 0.5f * (PsL + PsR);
+    if (SL <= 0.0f && Ss >= 0.0f) {
+      const float rs = L.r * (SL - unL) / (SL - Ss);
+      const float ux = Ss * nx + utL * tx, uy = Ss * ny + utL * ty;
+      const float Es = ((SL - unL) * WL.w - L.P * unL + Ps * Ss) / (SL - Ss);
+      F[0] = FL[0] + SL * (rs - WL.x);
+      F[1] = FL[1] + SL * (rs * ux - WL.y);
+      F[2] = FL[2] + SL * (rs * uy - WL.z);
+      F[3] = FL[3] + SL * (Es - WL.w);
+    } else if (Ss <= 0.0f && SR >= 0.0f) {
+      const float rs = R.r * (SR - unR) / (SR - Ss);
+      const float ux = Ss * nx + utR * tx, uy = Ss * ny + utR * ty;
+      const float Es = ((SR - unR) * WR.w - R.P * unR + Ps * Ss) / (SR - Ss);
+      F[0] = FR[0] + SR * (rs - WR.x);
+      F[1] = FR[1] + SR * (rs * ux - WR.y);
+      F[2] = FR[2] + SR * (rs * uy - WR.z);
+      F[3] = FR[3] + SR * (Es - WR.w);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) F[v] = FR[v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) D[v] = 0.5f * (FL[v] + FR[v]) - F[v];
+}
+
+__device__ __forceinline__ float wave_speed(const Prim& q, float nx, float ny, const Phys& ph) {
+  return sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM + fabsf(q.u * nx + q.v * ny);
+}
+
+// entropy variables (helper.py:48-70)
+__device__ __forceinline__ float4 entropy_vars(const float4 W, const Phys& ph) {
+  const float r = W.x, u = W.y / r, v = W.z / r;
+  const float P = (W.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
+  const float s = logf(P / powf(r, ph.g));
+  float4 V;
+  V.x = (ph.g - s) / ph.gm1 - 0.5f * r * (u * u + v * v) / P;
+  V.y = W.y / P;
+  V.z = W.z / P;
+  V.w = -r / P;
+  return V;
+}
+
+// d/dt getConserved_from_Entropy(eta + t deta) at t = 0 (the jax.jvp of
+// euler_solver.py:234-238), chain rule over helper.py:72-86.
+__device__ __forceinline__ void jvp_u_of_v(const float4 e, const float4 d, const Phys& ph, float out[4]) {
+  const float u = -e.y / e.w;
+  const float du = -(d.y / e.w - e.y * d.w / (e.w * e.w));
+  const float v = -e.z / e.w;
+  const float dv = -(d.z / e.w - e.z * d.w / (e.w * e.w));
+  const float q = 0.5f * (u * u + v * v);
+  const float dq = u * du + v * dv;
+  const float s = ph.g - ph.gm1 * (e.x - q * e.w);
+  const float ds = -ph.gm1 * (d.x - (dq * e.w + q * d.w));
+  const float a = powf(-e.w, ph.k_exp);
+  const float da = ph.k_exp * powf(-e.w, ph.k_exp - 1.0f) * (-d.w);
+  const float b = expf(s / ph.one_m_g);
+  const float db = b * ds / ph.one_m_g;
+  const float rho = a * b;
+  const float drho = da * b + a * db;
+  const float dP = -(drho / e.w - rho * d.w / (e.w * e.w));
+  const float kin = u * u + v * v;
+  const float dkin = 2.0f * u * du + 2.0f * v * dv;
+  out[0] = drho;
+  out[1] = drho * u + rho * du;
+  out[2] = drho * v + rho * dv;
+  out[3] = dP / ph.gm1 + 0.5f * (drho * kin + rho * dkin);
+}
+
+__device__ __forceinline__ float log_mean(float aL, float aR) {
+  const float xi = aL / aR;
+  const float f = (xi - 1.0f) / (xi + 1.0f);
+  const float u = f * f;
+  const float F = (u < 1e-3f) ? 1.0f + u / 3.0f + (u * u) / 5.0f + (u * (u * u)) / 7.0f : logf(xi) / (2.0f * f);
+  return (aL + aR) / (2.0f * F);
+}
+
+// Ismail-Roe entropy-conservative flux (euler_solver.py:532-588)
+__device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
+                                                float F[4]) {
+  float ZL[4], ZR[4];
+  {
+    const float r = WL.x, u = WL.y / r, v = WL.z / r;
+    const float P = (WL.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
+    const float s = sqrtf(r / P);
+    ZL[0] = s * 1.0f;
+    ZL[1] = s * u;
+    ZL[2] = s * v;
+    ZL[3] = s * P;
+  }
+  {
+    const float r = WR.x, u = WR.y / r, v = WR.z / r;
+    const float P = (WR.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
+    const float s = sqrtf(r / P);
+    ZR[0] = s * 1.0f;
+    ZR[1] = s * u;
+    ZR[2] = s * v;
+    ZR[3] = s * P;
+  }
+  float Zb[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Zb[v] = 0.5f * (ZL[v] + ZR[v]);
+  const float Z1 = log_mean(ZL[0], ZR[0]);
+  const float Z4 = log_mean(ZL[3], ZR[3]);
+  const float rh = Zb[0] * Z4;
+  const float uh = Zb[1] / Zb[0];
+  const float vh = Zb[2] / Zb[0];
+  const float P1 = Zb[3] / Zb[0];
+  const float P2 = ph.c_ir1 * Z4 / Z1 + ph.c_ir2 * Zb[3] / Zb[0];
+  const float ah = sqrtf(ph.g * P2 / rh);
+  const float Hh = ah * ah / ph.gm1 + (uh * uh + vh * vh) / 2.0f;
+  const float un = uh * nx + vh * ny;
+  F[0] = rh * un;
+  F[1] = rh * uh * un + P1 * nx;
+  F[2] = rh * vh * un + P1 * ny;
+  F[3] = rh * un * Hh;
+}
+
+__device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
+                                         float D[4]) {
+  // Roe average with clamps (helper.py:101-127)
+  float rL = WL.x;
+  rL = (rL <= 0.0f) ? 1e-6f : rL;
+  const float uL = WL.y / rL, vL = WL.z / rL;
+  float PL = (WL.w - 0.5f * rL * (uL * uL + vL * vL)) * ph.gm1;
+  PL = (PL <= 0.0f) ? 1e-6f : PL;
+  float rR = WR.x;
+  rR = (rR <= 0.0f) ? 1e-6f : rR;
+  const float uR = WR.y / rR, vR = WR.z / rR;
+  float PR = (WR.w - 0.5f * rR * (uR * uR + vR * vR)) * ph.gm1;
+  PR = (PR <= 0.0f) ? 1e-6f : PR;
+  const float sL = sqrtf(rL), sR = sqrtf(rR);
+  const float r = sL * sR;
+  const float u = (sL * uL + sR * uR) / (sL + sR);
+  const float v = (sL * vL + sR * vR) / (sL + sR);
+  const float HL = (WL.w + PL) / rL;
+  const float HR = (WR.w + PR) / rR;
+  const float H = (sL * HL + sR * HR) / (sL + sR);
+  const float a = sqrtf(fabsf(ph.gm1 * (H - 0.5f * (u * u + v * v))));
+  // eigenvectors R (columns), euler_solver.py:156-175
+  float Rm[4][4];
+  Rm[0][0] = 1.0f;
+  Rm[1][0] = u - a * nx;
+  Rm[2][0] = v - a * ny;
+  Rm[3][0] = H - a * (u * nx + v * ny);
+  Rm[0][1] = 1.0f;
+  Rm[1][1] = u;
+  Rm[2][1] = v;
+  Rm[3][1] = 0.5f * (u * u + v * v);
+  Rm[0][2] = 0.0f;
+  Rm[1][2] = -ny;
+  Rm[2][2] = nx;
+  Rm[3][2] = -u * ny + v * nx;
+  Rm[0][3] = 1.0f;
+  Rm[1][3] = u + a * nx;
+  Rm[2][3] = v + a * ny;
+  Rm[3][3] = H + a * (u * nx + v * ny);
+  const float un = u * nx + v * ny;
+  const float lam[4] = {un - a, un, un, un + a};
+  const float S[4] = {r / ph.two_g, ph.gm1 * r / ph.g, r / ph.g, r / ph.two_g};
+  const float4 eR = entropy_vars(WR, ph);
+  const float4 eL = entropy_vars(WL, ph);
+  const float dv[4] = {eR.x - eL.x, eR.y - eL.y, eR.z - eL.z, eR.w - eL.w};
+  float DRH[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float RH = ((Rm[0][i] * dv[0] + Rm[1][i] * dv[1]) + Rm[2][i] * dv[2]) + Rm[3][i] * dv[3];
+    DRH[i] = fabsf(lam[i] * S[i]) * RH;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    D[i] = 0.5f * (((Rm[i][0] * DRH[0] + Rm[i][1] * DRH[1]) + Rm[i][2] * DRH[2]) + Rm[i][3] * DRH[3]);
+}
+
+__device__ __forceinline__ void ausm_split(float Mn, float& Mp, float& Mm, float& Pp, float& Pm) {
+  const float absm = fabsf(Mn);
+  const bool sub = absm < 1.0f;
+  const float q = (Mn * Mn - 1.0f) * (Mn * Mn - 1.0f);
+  const float beta = 0.125f, alpha = 0.1875f;
+  Mp = sub ? 0.25f * ((Mn + 1.0f) * (Mn + 1.0f)) + beta * q : 0.5f * (Mn + absm);
+  Mm = sub ? -0.25f * ((Mn - 1.0f) * (Mn - 1.0f)) - beta * q : 0.5f * (Mn - absm);
+  const float sg = (Mn > 0.0f) ? 1.0f : ((Mn < 0.0f) ? -1.0f : 0.0f);
+  Pp = sub ? 0.25f * ((Mn + 1.0f) * (Mn + 1.0f)) * (2.0f - Mn) + alpha * Mn * q : 0.5f * (1.0f + sg);
+  Pm = sub ? 0.25f * ((Mn - 1.0f) * (Mn - 1.0f)) * (2.0f + Mn) - alpha * Mn * q : 0.5f * (1.0f - sg);
+}
+
+__device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float D[4]) {
+  const float unL = L.u * nx + L.v * ny;
+  const float unR = R.u * nx + R.v * ny;
+  const float aL = sqrtf(fabsf(ph.g * L.P / L.r)) * ph.invM;
+  const float aR = sqrtf(fabsf(ph.g * R.P / R.r)) * ph.invM;
+  const float af = 0.5f * (aL + aR);
+  const float MnL = unL / (aL + 1e-12f);
+  const float MnR = unR / (aR + 1e-12f);
+  float MpL, MmL_, PpL, PmL_, MpR_, MmR, PpR_, PmR;
+  ausm_split(MnL, MpL, MmL_, PpL, PmL_);
+  ausm_split(MnR, MpR_, MmR, PpR_, PmR);
+  const float mL = af * MpL * L.r;
+  const float mR = af * MmR * R.r;
+  const float pf = PpL * L.P + PmR * R.P;
+  const float HL = (WL.w + L.P * ph.invM2) / L.r;
+  const float HR = (WR.w + R.P * ph.invM2) / R.r;
+  float FA[4];
+  FA[0] = mL + mR;
+  FA[1] = mL * L.u + mR * R.u + pf * nx * ph.invM2;
+  FA[2] = mL * L.v + mR * R.v + pf * ny * ph.invM2;
+  FA[3] = mL * HL + mR * HR;
+  float FL[4], FR[4];
+  phys_flux(WL, L, unL, nx, ny, ph.invM2, FL);
+  phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) D[v] = 0.5f * (FL[v] + FR[v]) - FA[v];
+}
+
+template <int FLUX, int ENT>
+__device__ __forceinline__ void side_flux(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
+                                          float lf_c, float F[4]) {
+  const Prim L = prim_of(WL, ph.M2, ph.gm1);
+  const Prim R = prim_of(WR, ph.M2, ph.gm1);
+  float Fc[4], D[4];
+  constexpr bool IR = (FLUX == ESR_FLUX_TADMOR) || (ENT != 0);
+  if (IR)
+    ismail_roe_flux(WL, WR, nx, ny, ph, Fc);
+  else
+    central_flux(WL, L, WR, R, nx, ny, ph.invM2, Fc);
+
+  if (FLUX == ESR_FLUX_HLLC) {
+    diss_hllc(WL, L, WR, R, nx, ny, ph, D);
+  } else if (FLUX == ESR_FLUX_ROE) {
+    diss_roe(WL, WR, nx, ny, ph, D);
+  } else if (FLUX == ESR_FLUX_AUSM) {
+    diss_ausm(WL, L, WR, R, nx, ny, ph, D);
+  } else if (FLUX == ESR_FLUX_LF) {
+    const float h = 0.5f * lf_c;
+    D[0] = h * (WR.x - WL.x);
+    D[1] = h * (WR.y - WL.y);
+    D[2] = h * (WR.z - WL.z);
+    D[3] = h * (WR.w - WL.w);
+  } else {  // Rusanov; entropy-stable variant for Tadmor / entropy=True
+    const float Cm = fmaxf(wave_speed(R, nx, ny, ph), wave_speed(L, nx, ny, ph));
+    const float h = 0.5f * Cm;
+    if (FLUX == ESR_FLUX_TADMOR || ENT) {
+      const float4 eL = entropy_vars(WL, ph);
+      const float4 eR = entropy_vars(WR, ph);
+      const float4 eb = make_float4(0.5f * (eL.x + eR.x), 0.5f * (eL.y + eR.y), 0.5f * (eL.z + eR.z),
+                                    0.5f * (eL.w + eR.w));
+      const float4 de = make_float4(eR.x - eL.x, eR.y - eL.y, eR.z - eL.z, eR.w - eL.w);
+      float dU[4];
+      jvp_u_of_v(eb, de, ph, dU);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) D[v] = h * dU[v];
+    } else {
+      D[0] = h * (WR.x - WL.x);
+      D[1] = h * (WR.y - WL.y);
+      D[2] = h * (WR.z - WL.z);
+      D[3] = h * (WR.w - WL.w);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) F[v] = Fc[v] - D[v];
+}
+
+// Left and right conserved states of local face k of a work-item.
+template <int RECON>
+__device__ __forceinline__ void face_states(const Item& it, int k, uint32_t f, const float4 Wi, const CaseDev& cs,
+                                            const float4* __restrict__ Ws, const float4* __restrict__ rec,
+                                            float4& WL, float4& WR) {
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const Phys& ph = G->ph;
+  const int32_t n = m.n;
+  if (RECON == ESR_RECON_MUSCL) {
+    WL = cons_of(as_prim(rec[ridx_of(G, k, it.cell, it.c)]), ph.gm1M2);
+    if (fref_is_bnd(f)) {
+      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
+      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + it.cell] : make_float2(0.f, 0.f);
+      const float4 Wg = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws,
+                                    sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+      WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.gm1M2);
+    } else {
+      const int32_t j = (int32_t)(f & FREF_IDX);
+      const int kk = (int)(f >> 30);
+      WR = cons_of(as_prim(rec[ridx_of(G, kk, j, it.c)]), ph.gm1M2);
+    }
+  } else {
+    WL = Wi;
+    if (fref_is_bnd(f)) {
+      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
+      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + it.cell] : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+    } else {
+      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// flux + residual + stage update
+// ---------------------------------------------------------------------------
+
+template <int RECON, int FLUX, int ENT, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_flux(const GroupDev* __restrict__ groups,
+                                                const int32_t* __restrict__ blk_group,
+                                                const CaseDev* __restrict__ cases,
+                                                const float4* __restrict__ Ws, const float4* W0,
+                                                const float4* __restrict__ rec, float4* out, float4* acc) {
+  Item it;
+  if (!map_item(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const int32_t n = m.n, cell = it.cell;
+  const CaseDev& cs = cases[G->case0 + it.c];
+  const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
+
+  const float4 Wi = Ws[it.sidx];
+  float Fs[3][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t f = m.fref[(int64_t)k * n + cell];
+    float4 WL, WR;
+    face_states<RECON>(it, k, f, Wi, cs, Ws, rec, WL, WR);
+    const float2 nk = m.nrm[(int64_t)k * n + cell];
+    const float sk = m.len[(int64_t)k * n + cell];
+    float F[4];
+    side_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, G->ph, lf_c, F);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Fs[k][v] = sk * F[v];
+  }
+  const float ar = m.area[cell];
+  float R[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) R[v] = ((Fs[0][v] + Fs[1][v]) + Fs[2][v]) / ar;
+  if (G->sponge) {
+    const float sigma = cs.strength * G->sponge_g[cell];
+    R[0] = R[0] - sigma * (cs.value.x - Wi.x);
+    R[1] = R[1] - sigma * (cs.value.y - Wi.y);
+    R[2] = R[2] - sigma * (cs.value.z - Wi.z);
+    R[3] = R[3] - sigma * (cs.value.w - Wi.w);
+  }
+
+  const int64_t s = it.sidx;
+  if (MODE == MODE_RESID) {
+    out[s] = make_float4(R[0], R[1], R[2], R[3]);
+    return;
+  }
+  const float4 w0 = (MODE == MODE_EE) ? Wi : W0[s];
+  float4 o;
+  if (MODE == MODE_EE || MODE == MODE_S1) {
+    const float dt = cs.dt;
+    o = make_float4(w0.x - dt * R[0], w0.y - dt * R[1], w0.z - dt * R[2], w0.w - dt * R[3]);
+  } else if (MODE == MODE_SRK2_2) {
+    const float dt = cs.dt;
+    o = make_float4(0.5f * ((w0.x + Wi.x) - dt * R[0]), 0.5f * ((w0.y + Wi.y) - dt * R[1]),
+                    0.5f * ((w0.z + Wi.z) - dt * R[2]), 0.5f * ((w0.w + Wi.w) - dt * R[3]));
+  } else if (MODE == MODE_RK2_2) {
+    const float dt = cs.dt;
+    o = make_float4(0.5f * ((w0.x - dt * R[0]) + Wi.x), 0.5f * ((w0.y - dt * R[1]) + Wi.y),
+                    0.5f * ((w0.z - dt * R[2]) + Wi.z), 0.5f * ((w0.w - dt * R[3]) + Wi.w));
+  } else if (MODE == MODE_RK4_1) {
+    const float h = cs.dt_half;
+    o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
+    acc[s] = make_float4(R[0], R[1], R[2], R[3]);
+  } else if (MODE == MODE_RK4_2 || MODE == MODE_RK4_3) {
+    const float h = (MODE == MODE_RK4_2) ? cs.dt_half : cs.dt;
+    o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
+    const float4 a = acc[s];
+    acc[s] = make_float4(a.x + 2.0f * R[0], a.y + 2.0f * R[1], a.z + 2.0f * R[2], a.w + 2.0f * R[3]);
+  } else {  // MODE_RK4_4
+    const float h = cs.dt_sixth;
+    const float4 a = acc[s];
+    o = make_float4(w0.x - h * (a.x + R[0]), w0.y - h * (a.y + R[1]), w0.z - h * (a.z + R[2]),
+                    w0.w - h * (a.w + R[3]));
+  }
+  out[s] = o;
+}
+
+// Lax-Friedrichs: per-case maximum wave speed over every side-face state
+// (euler_solver.py:267-269), integer max on the float bits (values >= 0).
+template <int RECON>
+__global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ groups,
+                                                   const int32_t* __restrict__ blk_group, CaseDev* cases,
+                                                   const float4* __restrict__ Ws, const float4* __restrict__ rec) {
+  Item it;
+  if (!map_item(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const Phys& ph = G->ph;
+  CaseDev& cs = cases[G->case0 + it.c];
+  const float4 Wi = Ws[it.sidx];
+  float cm = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t f = m.fref[(int64_t)k * m.n + it.cell];
+    float4 WL, WR;
+    face_states<RECON>(it, k, f, Wi, cs, Ws, rec, WL, WR);
+    const float2 nk = m.nrm[(int64_t)k * m.n + it.cell];
+    const Prim L = prim_of(WL, ph.M2, ph.gm1), R = prim_of(WR, ph.M2, ph.gm1);
+    const float cl = wave_speed(L, nk.x, nk.y, ph), cr = wave_speed(R, nk.x, nk.y, ph);
+    const float c2 = (cl != cl) ? cl : ((cr != cr) ? cr : fmaxf(cl, cr));
+    cm = (c2 != c2) ? c2 : ((cm != cm) ? cm : fmaxf(cm, c2));
+  }
+  atomicMax(&cs.lf_cmax, __float_as_uint(cm));
+}
+
+// ---------------------------------------------------------------------------
+// per-case reductions
+// ---------------------------------------------------------------------------
+
+// Block layout of a reduction over one group: blockIdx.y = case chunk of up
+// to BLOCK cases, blockIdx.x = cell range. Thread t handles case c0 + t % Ck
+// and cells cs = t / Ck, cs + P, ... of the range (P = BLOCK / Ck lanes per
+// case); partials are combined over cs in a fixed order.
+struct RedMap {
+  int32_t c0, Ck, P, cell_begin, cell_end;
+};
+constexpr int32_t RED_ITERS = 8;
+
+__host__ __device__ inline int32_t red_cells_per_range(int32_t C) {
+  const int32_t Ck = C < BLOCK ? C : BLOCK;
+  return (BLOCK / Ck) * RED_ITERS;
+}
+
+int32_t reduce_ranges(const GroupDev& g) {
+  const int32_t cpr = red_cells_per_range(g.C);
+  return (g.m.n + cpr - 1) / cpr;
+}
+
+__device__ __forceinline__ RedMap red_map(const GroupDev* G) {
+  RedMap r;
+  r.c0 = blockIdx.y * BLOCK;
+  r.Ck = min(G->C - r.c0, BLOCK);
+  r.P = BLOCK / r.Ck;
+  const int32_t cpr = red_cells_per_range(G->C);
+  r.cell_begin = blockIdx.x * cpr;
+  r.cell_end = min(r.cell_begin + cpr, G->m.n);
+  return r;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ gdev, const CaseDev* __restrict__ cases,
+                                                 const float4* __restrict__ Wn, float4* __restrict__ Wref,
+                                                 double* __restrict__ part_d, double* __restrict__ part_r,
+                                                 int32_t rstride, int copy_ref) {
+  const GroupDev* G = gdev;
+  const RedMap r = red_map(G);
+  __shared__ double sd[BLOCK], sr[BLOCK];
+  const int t = threadIdx.x;
+  double accd = 0.0, accr = 0.0;
+  int32_t c = -1;
+  if (t < r.P * r.Ck) {
+    c = r.c0 + t % r.Ck;
+    const int32_t cs0 = t / r.Ck;
+    if (cases[G->case0 + c].check) {
+      for (int32_t cell = r.cell_begin + cs0; cell < r.cell_end; cell += r.P) {
+        const int64_t s = G->st_off + (int64_t)cell * G->C + c;
+        const float4 a = Wn[s];
+        const float4 b = Wref[s];
+        const double d0 = (double)a.x - (double)b.x, d1 = (double)a.y - (double)b.y;
+        const double d2 = (double)a.z - (double)b.z, d3 = (double)a.w - (double)b.w;
+        accd += ((d0 * d0 + d1 * d1) + d2 * d2) + d3 * d3;
+        accr += (((double)b.x * b.x + (double)b.y * b.y) + (double)b.z * b.z) + (double)b.w * b.w;
+        if (copy_ref) Wref[s] = a;
+      }
+    }
+  }
+  sd[t] = accd;
+  sr[t] = accr;
+  __syncthreads();
+  if (t < r.Ck) {
+    const int32_t cc = r.c0 + t;
+    double a = 0.0, b = 0.0;
+    for (int32_t q = 0; q < r.P; ++q) {
+      a += sd[q * r.Ck + t];
+      b += sr[q * r.Ck + t];
+    }
+    part_d[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = a;
+    part_r[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = b;
+  }
+}
+
+// helper.get_dt per cell: area / sum_k (c + |u.n_k|) len_k; min over cells.
+// local == 1 stores the per-cell value instead (local time stepping).
+__global__ __launch_bounds__(BLOCK) void k_dt(const GroupDev* __restrict__ gdev, const CaseDev* __restrict__ cases,
+                                              const float4* __restrict__ W, float* __restrict__ part,
+                                              int32_t rstride) {
+  const GroupDev* G = gdev;
+  const RedMap r = red_map(G);
+  const MeshDev& m = G->m;
+  const Phys& ph = G->ph;
+  __shared__ float sm[BLOCK];
+  const int t = threadIdx.x;
+  float acc = INFINITY;
+  if (t < r.P * r.Ck) {
+    const int32_t c = r.c0 + t % r.Ck;
+    const int32_t cs0 = t / r.Ck;
+    for (int32_t cell = r.cell_begin + cs0; cell < r.cell_end; cell += r.P) {
+      const Prim q = prim_of(W[G->st_off + (int64_t)cell * G->C + c], ph.M2, ph.gm1);
+      const float cel = sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM;
+      float tk[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float2 nk = m.nrm[(int64_t)k * m.n + cell];
+        const float un = q.u * nk.x + q.v * nk.y;
+        tk[k] = (cel + fabsf(un)) * m.len[(int64_t)k * m.n + cell];
+      }
+      const float d = m.area[cell] / ((tk[0] + tk[1]) + tk[2]);
+      acc = (d != d) ? d : ((acc != acc) ? acc : fminf(acc, d));
+    }
+  }
+  sm[t] = acc;
+  __syncthreads();
+  if (t < r.Ck) {
+    float a = INFINITY;
+    for (int32_t q = 0; q < r.P; ++q) {
+      const float x = sm[q * r.Ck + t];
+      a = (x != x) ? x : ((a != a) ? a : fminf(a, x));
+    }
+    part[(int64_t)(G->case0 + r.c0 + t) * rstride + blockIdx.x] = a;
+  }
+}
+
+// CD/CL wall sums in FP64 (helper.py:366-415). Pressure with the default
+// gamma = 1.4 and M = 1. One block per case of the group.
+__global__ __launch_bounds__(BLOCK) void k_forces(const GroupDev* __restrict__ gdev,
+                                                  const CaseDev* __restrict__ cases, const float4* __restrict__ W,
+                                                  double* __restrict__ cd_cl) {
+  const GroupDev* G = gdev;
+  const MeshDev& m = G->m;
+  const int32_t c = blockIdx.x;
+  __shared__ double sx[BLOCK], sy[BLOCK];
+  double ax = 0.0, ay = 0.0;
+  if (cases[G->case0 + c].check) {
+    for (int32_t w = threadIdx.x; w < m.n_wall; w += BLOCK) {
+      const int32_t cell = m.wall_cell[w];
+      const Prim q = prim_of(W[G->st_off + (int64_t)cell * G->C + c], 1.0f, 0.4f);
+      const double P = (double)q.P;
+      ax += P * (double)m.wall_nx[w] * (double)m.wall_ds[w];
+      ay += P * (double)m.wall_ny[w] * (double)m.wall_ds[w];
+    }
+  }
+  sx[threadIdx.x] = ax;
+  sy[threadIdx.x] = ay;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sx[threadIdx.x] += sx[threadIdx.x + s];
+      sy[threadIdx.x] += sy[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    cd_cl[2 * (int64_t)(G->case0 + c)] = sx[0];
+    cd_cl[2 * (int64_t)(G->case0 + c) + 1] = sy[0];
+  }
+}
+
+// Stationarity check of main.py:149-177 for every case whose check flag is set.
+__global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* __restrict__ part_d,
+                        const double* __restrict__ part_r, const int32_t* __restrict__ nranges, int32_t rstride,
+                        const double* __restrict__ forces, int64_t step, int32_t K) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cases) return;
+  CaseDev& cs = cases[c];
+  CaseLoop& L = loops[c];
+  if (!cs.check) return;
+  double d = 0.0, r = 0.0;
+  for (int32_t q = 0; q < nranges[c]; ++q) {
+    d += part_d[(int64_t)c * rstride + q];
+    r += part_r[(int64_t)c * rstride + q];
+  }
+  const int64_t block_steps = (step - L.ref_step) > 1 ? (step - L.ref_step) : 1;
+  const double rel = sqrt(d) / ((double)block_steps * (sqrt(r) + 1e-16));
+  const bool under = rel <= L.threshold;
+  const int32_t new_streak = under ? L.streak + 1 : 0;
+  const int64_t new_start = (under && L.streak == 0) ? step : L.streak_start;
+  const bool confirmed = (new_streak >= L.patience) && !L.stop;
+  if (confirmed) L.stopping_step = new_start;
+  L.stop = L.stop || confirmed;
+  L.streak = new_streak;
+  L.streak_start = new_start;
+  L.ref_step = step;
+  L.last_rel = rel;
+  if (!(d == d) || isinf(d)) L.diverged = 1;
+  if (L.track_eng && forces) {
+    const double cd = forces[2 * (int64_t)c] / L.q_ref, cl = forces[2 * (int64_t)c + 1] / L.q_ref;
+    L.cd = cd;
+    L.cl = cl;
+    const bool eu = (fabs(cd - L.target_cd) <= L.cd_tol) && (fabs(cl - L.target_cl) <= L.cl_tol);
+    const int32_t ns = eu ? L.eng_streak + 1 : 0;
+    const int64_t st = (eu && L.eng_streak == 0) ? step : L.eng_streak_start;
+    const bool ec = (ns >= L.eng_patience) && !L.eng_stop;
+    if (ec) L.eng_stopping_step = st;
+    L.eng_stop = L.eng_stop || ec;
+    L.eng_streak = ns;
+    L.eng_streak_start = st;
+  }
+  L.steps_run = step;
+  if (L.stop || step >= L.N) {
+    L.done = 1;
+    cs.active = 0;
+  }
+  cs.check = 0;
+}
+
+__global__ void k_dt_final(int32_t n_cases, CaseDev* cases, const float* __restrict__ part,
+                           const int32_t* __restrict__ nranges, int32_t rstride, const float* __restrict__ cfl,
+                           float* __restrict__ dt_out) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cases) return;
+  float a = INFINITY;
+  for (int32_t q = 0; q < nranges[c]; ++q) {
+    const float x = part[(int64_t)c * rstride + q];
+    a = (x != x) ? x : ((a != a) ? a : fminf(a, x));
+  }
+  const float dt = a * cfl[c];
+  cases[c].dt = dt;
+  cases[c].dt_half = dt / 2.0f;
+  cases[c].dt_sixth = dt / 6.0f;
+  dt_out[c] = dt;
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion and small utilities
+// ---------------------------------------------------------------------------
+
+__global__ void k_gather_in(const GroupDev* __restrict__ gdev, int32_t c, const int32_t* __restrict__ perm,
+                            const float4* __restrict__ user, float4* __restrict__ slab) {
+  const GroupDev* G = gdev;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G->m.n) return;
+  slab[G->st_off + (int64_t)i * G->C + c] = user[perm[i]];
+}
+
+__global__ void k_scatter_out(const GroupDev* __restrict__ gdev, int32_t c, const int32_t* __restrict__ perm,
+                              const float4* __restrict__ slab, float4* __restrict__ user) {
+  const GroupDev* G = gdev;
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= G->m.n) return;
+  user[perm[i]] = slab[G->st_off + (int64_t)i * G->C + c];
+}
+
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_set_active(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int32_t all) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cases) return;
+  cases[c].active = all ? 1 : (loops ? !loops[c].done : 1);
+  cases[c].check = 0;
+}
+
+__global__ void k_zero_lf(int32_t n_cases, CaseDev* cases) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n_cases) cases[c].lf_cmax = 0u;
+}
+
+__global__ void k_count_active(int32_t n_cases, const CaseDev* __restrict__ cases, int32_t* count) {
+  __shared__ int32_t s[BLOCK];
+  int32_t a = 0;
+  for (int32_t c = threadIdx.x; c < n_cases; c += BLOCK) a += cases[c].active ? 1 : 0;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int q = BLOCK / 2; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) s[threadIdx.x] += s[threadIdx.x + q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = s[0];
+}
+
+__global__ void k_set_case(CaseDev* dst, CaseDev v) { *dst = v; }
+__global__ void k_set_group(GroupDev* dst, GroupDev v) { *dst = v; }
+
+__global__ void k_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* __restrict__ loops, int64_t step,
+                             int32_t K, int32_t all) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cases) return;
+  if (all) {
+    cases[c].check = 1;
+    return;
+  }
+  cases[c].check = (cases[c].active && ((step % K) == 0 || step == loops[c].N)) ? 1 : 0;
+}
+
+__global__ void k_probe_final(int32_t n_cases, CaseDev* cases, const double* __restrict__ part_d,
+                              const double* __restrict__ part_r, const int32_t* __restrict__ nranges, int32_t rstride,
+                              double* __restrict__ out) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cases) return;
+  double d = 0.0, r = 0.0;
+  for (int32_t q = 0; q < nranges[c]; ++q) {
+    d += part_d[(int64_t)c * rstride + q];
+    r += part_r[(int64_t)c * rstride + q];
+  }
+  out[c] = sqrt(d) / (sqrt(r) + 1e-16);
+  cases[c].check = 0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_commit(const GroupDev* __restrict__ groups,
+                                                  const int32_t* __restrict__ blk_group,
+                                                  const CaseDev* __restrict__ cases, const float4* __restrict__ src,
+                                                  float4* __restrict__ dst) {
+  Item it;
+  if (!map_item(groups, blk_group, cases, it)) return;
+  dst[it.sidx] = src[it.sidx];
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+
+static void check_launch(const char* what) { hip_check(hipGetLastError(), what); }
+
+void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  hipLaunchKernelGGL(k_recon, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, Ws, rec);
+  check_launch("k_recon");
+}
+
+void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  CaseDev* cm = const_cast<CaseDev*>(cases);
+  if (ls.recon == ESR_RECON_MUSCL)
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cm,
+                       Ws, rec);
+  else
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group,
+                       cm, Ws, rec);
+  check_launch("k_lf_cmax");
+}
+
+template <int RECON, int FLUX, int ENT>
+static void launch_flux_mode(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                             const float4* rec, float4* out, float4* acc, hipStream_t st) {
+#define ESR_FLUX_CASE(MD)                                                                                    \
+  case MD:                                                                                                   \
+    hipLaunchKernelGGL((k_flux<RECON, FLUX, ENT, MD>), dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, \
+                       ls.blk_group, cases, Ws, W0, rec, out, acc);                                          \
+    break;
+  switch (mode) {
+    ESR_FLUX_CASE(MODE_RESID)
+    ESR_FLUX_CASE(MODE_EE)
+    ESR_FLUX_CASE(MODE_S1)
+    ESR_FLUX_CASE(MODE_SRK2_2)
+    ESR_FLUX_CASE(MODE_RK2_2)
+    ESR_FLUX_CASE(MODE_RK4_1)
+    ESR_FLUX_CASE(MODE_RK4_2)
+    ESR_FLUX_CASE(MODE_RK4_3)
+    ESR_FLUX_CASE(MODE_RK4_4)
+    default:
+      throw Error(ESR_ERR_VALUE, "unknown stage mode");
+  }
+#undef ESR_FLUX_CASE
+}
+
+template <int RECON, int ENT>
+static void launch_flux_f(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                          const float4* rec, float4* out, float4* acc, hipStream_t st) {
+  switch (ls.flux) {
+    case ESR_FLUX_LF: launch_flux_mode<RECON, ESR_FLUX_LF, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    case ESR_FLUX_RUSANOV:
+      launch_flux_mode<RECON, ESR_FLUX_RUSANOV, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      break;
+    case ESR_FLUX_TADMOR:
+      launch_flux_mode<RECON, ESR_FLUX_TADMOR, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      break;
+    case ESR_FLUX_AUSM: launch_flux_mode<RECON, ESR_FLUX_AUSM, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    case ESR_FLUX_HLLC: launch_flux_mode<RECON, ESR_FLUX_HLLC, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    case ESR_FLUX_ROE: launch_flux_mode<RECON, ESR_FLUX_ROE, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    default: throw Error(ESR_ERR_VALUE, "unknown flux");
+  }
+}
+
+void launch_flux(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                 const float4* rec, float4* out, float4* acc, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  if (ls.recon == ESR_RECON_MUSCL) {
+    if (ls.entropy)
+      launch_flux_f<ESR_RECON_MUSCL, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+    else
+      launch_flux_f<ESR_RECON_MUSCL, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+  } else {
+    if (ls.entropy)
+      launch_flux_f<ESR_RECON_CONSTANT, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+    else
+      launch_flux_f<ESR_RECON_CONSTANT, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+  }
+  check_launch("k_flux");
+}
+
+static dim3 red_grid(const GroupDev& g) {
+  return dim3((unsigned)reduce_ranges(g), (unsigned)((g.C + BLOCK - 1) / BLOCK));
+}
+
+void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn, float4* Wref,
+                  double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st) {
+  hipLaunchKernelGGL(k_norms, red_grid(g), dim3(BLOCK), 0, st, gdev, cases, Wn, Wref, part_d, part_r, rstride,
+                     copy_ref);
+  check_launch("k_norms");
+}
+
+void launch_dt(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, float* part,
+               int32_t rstride, int /*local*/, hipStream_t st) {
+  hipLaunchKernelGGL(k_dt, red_grid(g), dim3(BLOCK), 0, st, gdev, cases, W, part, rstride);
+  check_launch("k_dt");
+}
+
+void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, double* cd_cl,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(k_forces, dim3((unsigned)g.C), dim3(BLOCK), 0, st, gdev, cases, W, cd_cl);
+  check_launch("k_forces");
+}
+
+void launch_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* part_d, const double* part_r,
+                  const int32_t* nranges, int32_t rstride, const double* forces, int64_t step, int32_t K,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, part_d, part_r,
+                     nranges, rstride, forces, step, K);
+  check_launch("k_check");
+}
+
+void launch_dt_final(int32_t n_cases, CaseDev* cases, const float* part, const int32_t* nranges, int32_t rstride,
+                     const float* cfl, float* dt_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_dt_final, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, part, nranges,
+                     rstride, cfl, dt_out);
+  check_launch("k_dt_final");
+}
+
+void launch_gather_in_dev(int32_t n, const GroupDev* gdev, int32_t c, const int32_t* perm, const float4* user,
+                          float4* slab, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_in, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, gdev, c, perm, user, slab);
+  check_launch("k_gather_in");
+}
+
+void launch_scatter_out_dev(int32_t n, const GroupDev* gdev, int32_t c, const int32_t* perm, const float4* slab,
+                            float4* user, hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_out, dim3((n + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, gdev, c, perm, slab, user);
+  check_launch("k_scatter_out");
+}
+
+void launch_set_case(CaseDev* dst, const CaseDev& v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_case, dim3(1), dim3(1), 0, st, dst, v);
+  check_launch("k_set_case");
+}
+
+void launch_set_group(GroupDev* dst, const GroupDev& v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_group, dim3(1), dim3(1), 0, st, dst, v);
+  check_launch("k_set_group");
+}
+
+void launch_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int64_t step, int32_t K, int32_t all,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_mark_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, step, K, all);
+  check_launch("k_mark_check");
+}
+
+void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, const double* part_r,
+                        const int32_t* nranges, int32_t rstride, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_probe_final, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, part_d, part_r,
+                     nranges, rstride, out);
+  check_launch("k_probe_final");
+}
+
+void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src, float4* dst, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, src, dst);
+  check_launch("k_commit");
+}
+
+void launch_copy(const float4* src, float4* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, src, dst, n);
+  check_launch("k_copy");
+}
+
+void launch_set_active(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int32_t all, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_active, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, all);
+  check_launch("k_set_active");
+}
+
+void launch_zero_lf(int32_t n_cases, CaseDev* cases, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_lf, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases);
+  check_launch("k_zero_lf");
+}
+
+void launch_count_active(int32_t n_cases, const CaseDev* cases, int32_t* count, hipStream_t st) {
+  hipLaunchKernelGGL(k_count_active, dim3(1), dim3(BLOCK), 0, st, n_cases, cases, count);
+  check_launch("k_count_active");
+}
+
+}  // namespace esr

@@ -1,0 +1,68 @@
+// Launch wrappers of the gfx950 kernels (esr_kernels.hip), called by the
+// host orchestration (esr_api.cpp). Internal.
+#pragma once
+
+#include "esr_internal.h"
+
+namespace esr {
+
+// A launch covers a set of groups with the same kernel configuration.
+struct LaunchSet {
+  const GroupDev* groups = nullptr;   // device array
+  const int32_t* blk_group = nullptr; // device array: block -> group index
+  int32_t nblocks = 0;
+  int32_t flux = ESR_FLUX_HLLC;
+  int32_t recon = ESR_RECON_MUSCL;
+  int32_t entropy = 0;
+};
+
+constexpr int BLOCK = 256;
+
+void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st);
+void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec,
+                    hipStream_t st);
+void launch_flux(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                 const float4* rec, float4* out, float4* acc, hipStream_t st);
+
+// Per-case reductions over one group (host-side group descriptor `g`,
+// device copy at `gdev`). Partial results at part[case * rstride + r].
+int32_t reduce_ranges(const GroupDev& g);
+void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn,
+                  float4* Wref, double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st);
+void launch_dt(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, float* part,
+               int32_t rstride, int local, hipStream_t st);
+void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W,
+                   double* cd_cl, hipStream_t st);
+
+// Finalisation of a stationarity check for every case (main.py:149-177).
+void launch_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* part_d,
+                  const double* part_r, const int32_t* nranges, int32_t rstride, const double* forces,
+                  int64_t step, int32_t K, hipStream_t st);
+// dt finalisation: dt = min * cfl; writes dt, dt/2, dt/6 into the case params.
+void launch_dt_final(int32_t n_cases, CaseDev* cases, const float* part, const int32_t* nranges,
+                     int32_t rstride, const float* cfl, float* dt_out, hipStream_t st);
+
+// Layout conversion between a user array (original order, (n,4)) and a
+// group slab (renumbered, case-innermost).
+void launch_gather_in_dev(int32_t n, const GroupDev* gdev, int32_t c, const int32_t* perm, const float4* user,
+                          float4* slab, hipStream_t st);
+void launch_scatter_out_dev(int32_t n, const GroupDev* gdev, int32_t c, const int32_t* perm, const float4* slab,
+                            float4* user, hipStream_t st);
+// stage-ordered parameter updates (kernel arguments are copied at launch)
+void launch_set_case(CaseDev* dst, const CaseDev& v, hipStream_t st);
+void launch_set_group(GroupDev* dst, const GroupDev& v, hipStream_t st);
+// check flags: all == 1 marks every case, otherwise active cases with
+// step % K == 0 or step == N.
+void launch_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int64_t step, int32_t K, int32_t all,
+                       hipStream_t st);
+// final one-step probe residual (main.py:228-230)
+void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, const double* part_r,
+                        const int32_t* nranges, int32_t rstride, double* out, hipStream_t st);
+// commit out -> W0 for the active work-items of a launch set (explicit Euler)
+void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src, float4* dst, hipStream_t st);
+void launch_copy(const float4* src, float4* dst, int64_t n, hipStream_t st);
+void launch_set_active(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int32_t all, hipStream_t st);
+void launch_zero_lf(int32_t n_cases, CaseDev* cases, hipStream_t st);
+void launch_count_active(int32_t n_cases, const CaseDev* cases, int32_t* count, hipStream_t st);
+
+}  // namespace esr

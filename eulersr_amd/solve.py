@@ -1,0 +1,135 @@
+"""Batched solve loop on the GPU (esr_solver_* of libeulersr.so).
+
+One ``BatchSolver`` advances any number of (mesh, Mach, alpha, flux) cases
+together: cases on the same mesh with the same kernel configuration share a
+group whose state is stored case-innermost, so a wave reads one geometry
+record for up to 64 cases. Each case keeps the reference loop semantics
+(euler/main.py:116-230): its own global dt from its initial state, its own
+N = int(tf/dt)+1, its own stationarity streak, frozen once stopped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .device import as_device_state, device_mesh, require_gpu, stream_ptr
+from .euler_solver import SCHEME, make_opts
+
+
+@dataclass
+class CaseSpec:
+    mesh: object
+    kw: dict                       # solver kwargs (flux, reconstruction, gamma, value, ...)
+    cfl: float = 0.6               # after the transonic clamp
+    tf: float = 4.0
+    q_ref: float = 0.0             # 0.5 rho U^2 L (0: no forces)
+    engineering: tuple | None = None  # (target_cd, target_cl, cd_tol_abs, cl_tol_abs, patience)
+    tag: dict = field(default_factory=dict)
+
+
+def transonic_cfl(mach: float, cfl: float) -> float:
+    """main.py:117-120"""
+    return 0.45 if (0.6 < mach < 1.1 and cfl > 0.45) else cfl
+
+
+class BatchSolver:
+    def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0):
+        require_gpu()
+        self.specs = list(cases)
+        n = len(self.specs)
+        arr = (L.Case * n)()
+        self._meshes = []
+        for i, cs in enumerate(self.specs):
+            dm = device_mesh(cs.mesh, cs.kw)
+            self._meshes.append(dm)
+            arr[i].mesh = dm.handle
+            arr[i].opts = make_opts(cs.kw)
+            arr[i].cfl = float(cs.cfl)
+            arr[i].tf = float(cs.tf)
+            arr[i].q_ref = float(cs.q_ref)
+            if cs.engineering is not None:
+                tcd, tcl, cdt, clt, pat = cs.engineering
+                arr[i].track_engineering = 1
+                arr[i].target_cd, arr[i].target_cl = float(tcd), float(tcl)
+                arr[i].cd_tol_abs, arr[i].cl_tol_abs = float(cdt), float(clt)
+                arr[i].engineering_patience = int(pat)
+        so = L.SolveOpts()
+        so.scheme = SCHEME[str(scheme).upper()]
+        so.threshold = float(threshold)
+        so.check_every = max(1, int(check_every))
+        so.patience = max(1, int(patience))
+        so.max_steps = int(max_steps)
+        so.local_dt = 0
+        h = C.c_void_p()
+        L.check(L.lib().esr_solver_create(n, arr, C.byref(so), C.byref(h)), "esr_solver_create")
+        self.handle = h
+        self.n = n
+        self.steps = 0
+
+    def set_state(self, i, W):
+        t, _ = as_device_state(W)
+        if t.shape != (self.specs[i].mesh.tris.shape[0], 4):
+            raise ValueError(f"state of case {i} has shape {tuple(t.shape)}")
+        L.check(L.lib().esr_solver_set_state(self.handle, i, C.c_void_p(t.data_ptr()), stream_ptr()),
+                "esr_solver_set_state")
+        # keep the source alive until the copy has run
+        import torch
+        torch.cuda.current_stream().synchronize()
+
+    def get_state(self, i):
+        import torch
+        n = int(self.specs[i].mesh.tris.shape[0])
+        out = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        L.check(L.lib().esr_solver_get_state(self.handle, i, C.c_void_p(out.data_ptr()), stream_ptr()),
+                "esr_solver_get_state")
+        return out
+
+    def prepare(self):
+        L.check(L.lib().esr_solver_prepare(self.handle, stream_ptr()), "esr_solver_prepare")
+
+    def run(self):
+        s = C.c_int64()
+        L.check(L.lib().esr_solver_run(self.handle, C.byref(s), stream_ptr()), "esr_solver_run")
+        self.steps = s.value
+        return self.steps
+
+    def finalize(self):
+        L.check(L.lib().esr_solver_finalize(self.handle, stream_ptr()), "esr_solver_finalize")
+
+    def report(self, i) -> dict:
+        r = L.CaseReport()
+        L.check(L.lib().esr_solver_report(self.handle, i, C.byref(r)), "esr_solver_report")
+        return dict(dt=np.float32(r.dt), N=int(r.n_steps), converged=bool(r.converged),
+                    stopping_step=int(r.stopping_step), steps_run=int(r.steps_run),
+                    last_rel=float(r.last_rel), final_residual=float(r.final_residual),
+                    cd=float(r.cd), cl=float(r.cl),
+                    engineering_converged=bool(r.engineering_converged),
+                    engineering_stopping_step=int(r.engineering_stopping_step),
+                    diverged=bool(r.diverged))
+
+    def info(self):
+        items, cells = C.c_int64(), C.c_int64()
+        L.check(L.lib().esr_solver_info(self.handle, C.byref(items), C.byref(cells)), "esr_solver_info")
+        return items.value, cells.value
+
+    def time_kernels(self, reps=20):
+        """Average ms per launch of (reconstruction, stage-1 flux) kernels,
+        HIP events on the current stream."""
+        a, b = C.c_double(), C.c_double()
+        L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), C.byref(a), C.byref(b), stream_ptr()),
+                "esr_solver_time_kernels")
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            L.lib().esr_solver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,200 @@
+/*
+ * eulersr.h -- C ABI of the MI355X explicit 2D unstructured finite-volume
+ * Euler solver (libeulersr.so, gfx950).
+ *
+ * The reference exposes no FFI: its boundary is a set of Python functions
+ * (euler/jax_fvm/src/euler_solver.py, helper.py, euler/main.py). Each entry
+ * point below names the reference function it replaces; the Python package
+ * eulersr_amd binds them with ctypes and keeps the reference signatures.
+ *
+ * Conventions
+ *   - State arrays W are float32, shape (n_cells, 4) row-major, conserved
+ *     variables (rho, rho u, rho v, E), cells in the mesh's ORIGINAL order.
+ *     Pointers passed to compute entry points are DEVICE pointers; the
+ *     library renumbers internally (reverse Cuthill-McKee) and undoes it.
+ *   - Mesh descriptions are HOST pointers; the library copies them.
+ *   - `stream` is a hipStream_t (NULL = default stream). Entry points that
+ *     return host values synchronise that stream.
+ *   - Every entry point returns an esr_status; esr_last_error() gives the
+ *     message of the calling thread's last failure.
+ *   - Handles are not thread-safe; use one handle per host thread.
+ */
+#ifndef EULERSR_H
+#define EULERSR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  ESR_OK = 0,
+  ESR_ERR_VALUE = 1,    /* bad argument (Python ValueError)            */
+  ESR_ERR_KEY = 2,      /* unknown option key (Python KeyError)        */
+  ESR_ERR_DEVICE = 3,   /* HIP runtime failure (Python RuntimeError)   */
+  ESR_ERR_MEMORY = 4,   /* allocation failure (Python MemoryError)     */
+  ESR_ERR_STATE = 5     /* call not valid in the handle's state        */
+} esr_status;
+
+/* numerical flux: euler_solver.py:124-139 ('hll' maps to HLLC) */
+typedef enum {
+  ESR_FLUX_LF = 0, ESR_FLUX_RUSANOV = 1, ESR_FLUX_TADMOR = 2,
+  ESR_FLUX_AUSM = 3, ESR_FLUX_HLLC = 4, ESR_FLUX_ROE = 5
+} esr_flux;
+
+/* reconstruction: euler_solver.py:106-122 */
+typedef enum { ESR_RECON_CONSTANT = 0, ESR_RECON_MUSCL = 1 } esr_recon;
+
+/* time scheme: euler_solver.py:630-667 (dispatch table main.py:50-51) */
+typedef enum {
+  ESR_SCHEME_EE = 0, ESR_SCHEME_RK2 = 1, ESR_SCHEME_SRK2 = 2, ESR_SCHEME_RK4 = 3
+} esr_scheme;
+
+typedef struct esr_mesh esr_mesh;
+typedef struct esr_solver esr_solver;
+
+/* Host description of a mesh, fields of the reference Mesh pytree
+ * (jax_fvm/src/mesh.py:28-83); geometry already cast to float32. */
+typedef struct {
+  int32_t n_points, n_cells, n_faces;
+  const float* points;                      /* (n_points,2)              */
+  const int32_t* tris;                      /* (n_cells,3)               */
+  const int32_t* neighbors;                 /* (n_cells,3), -1 boundary  */
+  const int32_t* faces;                     /* (n_faces,2)               */
+  const int32_t* face_markers;              /* (n_faces,)                */
+  const int32_t* face_connectivity;         /* (n_cells,3)               */
+  const int32_t* face_connectivity_opposite;/* (n_faces,)                */
+  const float* barycenter;                  /* (n_cells,2)               */
+  const float* area;                        /* (n_cells,)                */
+  const float* surface;                     /* (n_faces,)                */
+  const float* normals;                     /* (n_cells,3,2)             */
+  const float* inlet_subsonic;              /* (4,) primitives           */
+  int32_t n_wall_markers;                   /* helper.py:237             */
+  const int32_t* wall_markers;
+  int32_t force_marker;                     /* helper.py:370 (default 2) */
+  int32_t sponge;                           /* 0 when metadata case=='bump' (helper.py:266) */
+  int32_t renumber;                         /* 1: reverse Cuthill-McKee  */
+} esr_mesh_desc;
+
+/* Solver keyword arguments of residual()/time_step_*() (euler_solver.py:602-667,
+ * helper.py:262-295). sponge_width/strength < 0 select the defaults. */
+typedef struct {
+  int32_t flux;          /* esr_flux  */
+  int32_t recon;         /* esr_recon */
+  int32_t entropy;       /* 1: Ismail-Roe central flux (getFlux_Tadmor) */
+  double gamma;          /* double: derived constants (gamma-1, ...) are   */
+  double M;              /* rounded to float32 once, as Python does        */
+  float value[4];        /* inlet conserved state ('value' kwarg) */
+  float sponge_width;
+  float sponge_strength;
+} esr_opts;
+
+/* -- mesh ------------------------------------------------------------------ */
+
+/* Replaces Mesh.mesh_generator_from_points + load_mesh (mesh.py:131-150,
+ * 337-359) on the device side: validates, renumbers (RCM) and uploads. */
+esr_status esr_mesh_create(const esr_mesh_desc* desc, esr_mesh** out);
+esr_status esr_mesh_destroy(esr_mesh* mesh);
+/* perm[new] = old cell index (identity when renumber == 0). */
+esr_status esr_mesh_permutation(const esr_mesh* mesh, int32_t* perm);
+esr_status esr_mesh_info(const esr_mesh* mesh, int32_t* n_cells, int32_t* n_wall_faces,
+                         int64_t* device_bytes);
+/* Host-only: the reverse Cuthill-McKee order esr_mesh_create uses, from a
+ * (n_cells,3) neighbour table (-1 = boundary). perm[new] = old. No GPU needed.
+ * No reference counterpart (the reference keeps the mesher's order). */
+esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, int32_t* perm);
+
+/* -- single-state operations (device W, original cell order) -------------- */
+
+/* residual(W, mesh, **kw) -- euler_solver.py:602-628. R = sum(F*len)/area - sponge. */
+esr_status esr_residual(esr_mesh* mesh, const esr_opts* opts, const float* W, float* R,
+                        void* stream);
+
+/* time_step_{Euler,RK2,RK2_SSP,RK4}(W, mesh, dt, **kw) -- euler_solver.py:630-667.
+ * W_out may equal W. */
+esr_status esr_step(esr_mesh* mesh, const esr_opts* opts, int32_t scheme, float dt,
+                    const float* W, float* W_out, void* stream);
+
+/* helper.get_dt(W, mesh, CFL, gamma, M) -- helper.py:8-18 (synchronises). */
+esr_status esr_get_dt(esr_mesh* mesh, const float* W, float cfl, double gamma, double M,
+                      float* dt_out, void* stream);
+
+/* get_drag_coefficient / get_lift_coefficient -- helper.py:366-415, wall sums in
+ * float64; q_ref = 0.5 rho_inf U_inf^2 L_ref (synchronises). */
+esr_status esr_forces(esr_mesh* mesh, const float* W, double q_ref, double* cd, double* cl,
+                      void* stream);
+
+/* ||A - B||_F and ||B||_F in float64 over n_cells x 4 (the norms of main.py:152, 230). */
+esr_status esr_norms(esr_mesh* mesh, const float* A, const float* B, double* diff_norm,
+                     double* ref_norm, void* stream);
+
+/* -- batched solve loop (main.py:116-230 for many cases) ------------------ */
+
+typedef struct {
+  const esr_mesh* mesh;
+  esr_opts opts;
+  float cfl;                 /* after the transonic clamp of main.py:117-120 */
+  float tf;                  /* N = int(tf/dt) + 1 (main.py:122)             */
+  double q_ref;              /* 0.5 rho U^2 L for CD/CL (<= 0: no forces)    */
+  int32_t track_engineering; /* engineering CD/CL criterion (main.py:66-114) */
+  double target_cd, target_cl, cd_tol_abs, cl_tol_abs;
+  int32_t engineering_patience;
+} esr_case;
+
+typedef struct {
+  int32_t scheme;            /* esr_scheme, shared by all cases */
+  double threshold;          /* stationarity_threshold          */
+  int32_t check_every;       /* stationarity_check_every        */
+  int32_t patience;          /* stationarity_patience (>= 1)    */
+  int64_t max_steps;         /* extra cap on steps (<= 0: none) */
+  int32_t local_dt;          /* 0: global dt (reference). 1: local CFL dt (acceleration mode) */
+} esr_solve_opts;
+
+typedef struct {
+  float dt;
+  int64_t n_steps;           /* N */
+  int32_t converged;
+  int64_t stopping_step;
+  int64_t steps_run;
+  double last_rel;
+  double final_residual;     /* one-step probe, main.py:228-230 */
+  double cd, cl;
+  int32_t engineering_converged;
+  int64_t engineering_stopping_step;
+  int32_t diverged;          /* non-finite state detected */
+} esr_case_report;
+
+esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_solve_opts* so,
+                             esr_solver** out);
+esr_status esr_solver_destroy(esr_solver* s);
+/* Initial state of case c (device pointer, original cell order, (n,4) float32). */
+esr_status esr_solver_set_state(esr_solver* s, int32_t c, const float* W, void* stream);
+esr_status esr_solver_get_state(esr_solver* s, int32_t c, float* W, void* stream);
+/* Computes dt per case from the current states (helper.py:8-18 once per run,
+ * main.py:121) and N = int(tf/dt)+1. */
+esr_status esr_solver_prepare(esr_solver* s, void* stream);
+/* Advances until every case has stopped or reached N (or max_steps);
+ * returns the number of steps taken in *steps. */
+esr_status esr_solver_run(esr_solver* s, int64_t* steps, void* stream);
+/* One-step probe residual + CD/CL of the current states. */
+esr_status esr_solver_finalize(esr_solver* s, void* stream);
+esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out);
+
+/* -- diagnostics ----------------------------------------------------------- */
+
+/* Work-items (cell x case pairs) and distinct group cells of one stage launch. */
+esr_status esr_solver_info(const esr_solver* s, int64_t* work_items, int64_t* group_cells);
+/* Average device time (ms) of one launch of the reconstruction kernel and of
+ * the stage-1 flux/update kernel over the solver's current states, each
+ * launched `reps` times back to back between two HIP events on `stream`.
+ * Both launches are idempotent (they rewrite scratch and the stage buffer). */
+esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms_recon, double* ms_flux,
+                                   void* stream);
+const char* esr_last_error(void);
+int32_t esr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EULERSR_H */

@@ -1,0 +1,240 @@
+"""Parity of the gfx950 path (through the C ABI) with the CPU oracle.
+
+Floating-point tolerance rule: the HIP result may differ from the float32
+oracle by at most 4x the float32-vs-float64 oracle difference of the same
+quantity (the rounding noise of the reference arithmetic itself), with a
+floor of 1e-6 of the quantity's magnitude. The north-star bar (per-cell
+primitives within max relative 1e-5 after 100 steps, CD/CL within 1e-4) is
+tested on its own below. Integer/index work (renumbering, batching) must be
+bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import base_cfg, mesh_of, perturbed_state
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = ["hllc", "roe", "rusanov", "lf", "ausm", "tadmor"]
+RECONS = ["constant", "muscl"]
+
+
+def _oracle():
+    from oracle import fvm as F
+    return F
+
+
+def _kw(inlet, flux, recon, **extra):
+    kw = dict(gamma=1.4, M=1.0, reconstruction=recon, flux=flux, value=inlet, entropy=False)
+    kw.update(extra)
+    return kw
+
+
+def assert_noise_close(gpu, r32, r64, what, factor=4.0, floor=1e-6):
+    gpu = np.asarray(gpu, dtype=np.float64)
+    r32 = np.asarray(r32, dtype=np.float64)
+    r64 = np.asarray(r64, dtype=np.float64)
+    assert np.all(np.isfinite(gpu)), f"{what}: non-finite GPU values"
+    for v in range(gpu.shape[-1]):
+        err = np.max(np.abs(gpu[..., v] - r32[..., v]))
+        noise = np.max(np.abs(r32[..., v] - r64[..., v]))
+        mag = np.max(np.abs(r64[..., v]))
+        tol = max(factor * noise, floor * mag)
+        print(f"{what} var{v}: err {err:.3e} noise {noise:.3e} mag {mag:.3e}")
+        assert err <= tol, f"{what} var {v}: |gpu-oracle32| {err:.3e} > tol {tol:.3e} (noise {noise:.3e})"
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("recon", RECONS)
+def test_residual_matches_oracle(flux, recon):
+    from eulersr_amd import euler_solver
+    F = _oracle()
+    mesh = mesh_of("diamond", 0.2)
+    cfg = base_cfg(Mach=1.8)
+    W, inlet = perturbed_state(mesh, cfg)
+    kw = _kw(inlet, flux, recon)
+    print(f"case: residual diamond h=0.2 flux={flux} recon={recon}", flush=True)
+    Rg = euler_solver.residual(W, mesh, **kw)
+    R32 = F.residual(W, F.omesh(mesh, np.float32), **kw)
+    kw64 = dict(kw, value=inlet.astype(np.float64))
+    R64 = F.residual(W.astype(np.float64), F.omesh(mesh, np.float64), **kw64)
+    assert_noise_close(Rg, R32, R64, f"residual {flux}/{recon}")
+
+
+@pytest.mark.parametrize("flux", ["hllc", "rusanov"])
+def test_entropy_flag_residual(flux):
+    from eulersr_amd import euler_solver
+    F = _oracle()
+    mesh = mesh_of("naca0012", 0.2)
+    cfg = base_cfg(Mach=0.8, aoa=2.0, case="naca0012")
+    W, inlet = perturbed_state(mesh, cfg, seed=3)
+    kw = _kw(inlet, flux, "muscl", entropy=True)
+    print(f"case: entropy residual naca0012 flux={flux}", flush=True)
+    Rg = euler_solver.residual(W, mesh, **kw)
+    R32 = F.residual(W, F.omesh(mesh, np.float32), **kw)
+    R64 = F.residual(W.astype(np.float64), F.omesh(mesh, np.float64), **dict(kw, value=inlet.astype(np.float64)))
+    assert_noise_close(Rg, R32, R64, f"entropy residual {flux}")
+
+
+@pytest.mark.parametrize("scheme", ["EE", "RK2", "SRK2", "RK4"])
+def test_one_step_each_scheme(scheme):
+    from eulersr_amd import euler_solver
+    F = _oracle()
+    mesh = mesh_of("diamond", 0.2)
+    cfg = base_cfg()
+    W, inlet = perturbed_state(mesh, cfg, seed=1)
+    kw = _kw(inlet, "hllc", "muscl")
+    om32 = F.omesh(mesh, np.float32)
+    dt = F.get_dt(W, om32, CFL=0.5)
+    print(f"case: one {scheme} step dt={dt}", flush=True)
+    Wg = euler_solver.time_step(scheme, W, mesh, dt, **kw)
+    W32 = F.SCHEMES[scheme](W, om32, dt, **kw)
+    W64 = F.SCHEMES[scheme](W.astype(np.float64), F.omesh(mesh, np.float64), np.float64(dt),
+                            **dict(kw, value=inlet.astype(np.float64)))
+    assert_noise_close(Wg - W, W32 - W, W64 - W.astype(np.float64), f"step {scheme} increment")
+
+
+def test_get_dt_matches_oracle():
+    from eulersr_amd import helper
+    F = _oracle()
+    mesh = mesh_of("naca0012", 0.2)
+    cfg = base_cfg(case="naca0012", Mach=0.9)
+    W, _ = perturbed_state(mesh, cfg, seed=2)
+    dg = float(helper.get_dt(W, mesh, CFL=0.45, gamma=1.4, M=1.0))
+    d32 = float(F.get_dt(W, F.omesh(mesh, np.float32), CFL=0.45))
+    print(f"case: get_dt gpu {dg!r} oracle {d32!r}", flush=True)
+    assert abs(dg - d32) <= 2e-7 * abs(d32)
+
+
+def test_forces_match_oracle():
+    from eulersr_amd import helper
+    F = _oracle()
+    mesh = mesh_of("diamond", 0.2)
+    cfg = base_cfg()
+    W, _ = perturbed_state(mesh, cfg, seed=4)
+    U = 2.5 * np.sqrt(1.4)
+    cd, cl = helper.get_force_coefficients(W, mesh, 1.0, U, 1.0)
+    rcd, rcl = F.forces(W, F.omesh(mesh, np.float32), 1.0, U, 1.0)
+    print(f"case: forces gpu ({cd}, {cl}) oracle ({rcd}, {rcl})", flush=True)
+    assert abs(cd - rcd) <= 1e-6 * max(abs(rcd), 1e-3)
+    assert abs(cl - rcl) <= 1e-6 * max(abs(rcl), 1e-3)
+
+
+def test_100_steps_primitives_within_1e5():
+    """per-cell primitives within max relative 1e-5 after 100
+    SRK2 steps (HLLC + MUSCL), relative to max(|p|, freestream scale)."""
+    from eulersr_amd import euler_solver
+    F = _oracle()
+    mesh = mesh_of("diamond", 0.1)
+    cfg = base_cfg(Mach=2.5, aoa=5.0)
+    W0, inlet = F.initialize(mesh.n_cells, cfg, np.float32)
+    kw = _kw(inlet, "hllc", "muscl")
+    om = F.omesh(mesh, np.float32)
+    dt = F.get_dt(W0, om, CFL=0.6)
+    import torch
+    Wg = torch.from_numpy(W0).cuda()
+    Wr = W0.copy()
+    print("case: 100 SRK2 steps diamond h=0.1", flush=True)
+    for _ in range(100):
+        Wg = euler_solver.time_step_RK2_SSP(Wg, mesh, dt, **kw)
+        Wr = F.time_step_RK2_SSP(Wr, om, dt, **kw)
+    pg = F.getPrimitive(Wg.cpu().numpy().astype(np.float64))
+    pr = F.getPrimitive(Wr.astype(np.float64))
+    U = np.hypot(inlet[1], inlet[2]) / inlet[0]
+    scale = np.array([1.0, U, U, 1.0])
+    rel = np.abs(pg - pr) / np.maximum(np.abs(pr), scale)
+    print("max rel per primitive:", rel.max(axis=0))
+    assert rel.max() <= 1e-5
+
+
+def test_renumbering_is_bit_invariant():
+    """RCM renumbering changes memory order only: results are bit-identical."""
+    import torch
+    from eulersr_amd import _lib as L
+    from eulersr_amd.device import DeviceMesh
+    from eulersr_amd.euler_solver import make_opts
+    import ctypes as C
+    mesh = mesh_of("naca0012", 0.2)
+    cfg = base_cfg(case="naca0012", Mach=1.3)
+    W, inlet = perturbed_state(mesh, cfg, seed=5)
+    opts = make_opts(_kw(inlet, "hllc", "muscl"))
+    outs = []
+    for ren in (True, False):
+        dm = DeviceMesh(mesh, wall_markers=(2,), renumber=ren)
+        t = torch.from_numpy(W).cuda()
+        o = torch.empty_like(t)
+        L.check(L.lib().esr_residual(dm.handle, C.byref(opts), C.c_void_p(t.data_ptr()), C.c_void_p(o.data_ptr()),
+                                     None))
+        torch.cuda.synchronize()
+        outs.append(o.cpu().numpy())
+        if ren:
+            perm = dm.permutation()
+            assert not np.array_equal(perm, np.arange(mesh.n_cells))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
+def test_bitwise_repeatable():
+    from eulersr_amd import euler_solver
+    mesh = mesh_of("diamond", 0.2)
+    cfg = base_cfg()
+    W, inlet = perturbed_state(mesh, cfg, seed=6)
+    kw = _kw(inlet, "roe", "muscl")
+    a = euler_solver.time_step_RK4(W, mesh, 1e-3, **kw)
+    b = euler_solver.time_step_RK4(W, mesh, 1e-3, **kw)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_solve_loop_matches_oracle():
+    """Stationarity loop (check every K, patience, freeze, probe) against the
+    oracle's restatement of main.py on a small case."""
+    from eulersr_amd import main as M
+    F = _oracle()
+    mesh = mesh_of("diamond", 0.2)
+    cfg = base_cfg(Mach=2.5, aoa=5.0, tf=1.0, stationarity_threshold=1e-3, stationarity_check_every=5,
+                   stationarity_patience=2, verbose=False, export={})
+    W, inlet = M.initialize(mesh, cfg)
+    W_or, inlet_or = F.initialize(mesh.n_cells, cfg, np.float32)
+    assert np.array_equal(W, W_or) and np.array_equal(inlet, inlet_or)
+    print("case: solve loop diamond h=0.2 threshold 2e-3 patience 2", flush=True)
+    Wg = M.run(W, mesh, inlet, cfg, out_dirs=None)
+    rep = cfg["_last_run"]["report"]
+    Wr, info = F.run(W_or, F.omesh(mesh, np.float32), inlet_or, cfg)
+    print(rep, info)
+    assert rep["N"] == info["N"]
+    assert abs(float(rep["dt"]) - float(info["dt"])) <= 2e-7 * float(info["dt"])
+    assert rep["converged"] == info["converged"]
+    assert rep["stopping_step"] == info["stopping_step"]
+    assert abs(rep["final_residual"] - info["final_residual"]) <= 1e-3 * info["final_residual"]
+    d = np.abs(Wg.astype(np.float64) - Wr) / np.maximum(np.abs(Wr), 1.0)
+    assert d.max() < 1e-4
+
+
+def test_batch_equals_single_cases_bitwise():
+    """Cases batched on one mesh (case-innermost layout) and on two meshes in
+    one launch give bit-identical states to the same cases run alone."""
+    from eulersr_amd import main as M
+    from eulersr_amd.solve import BatchSolver
+    m1 = mesh_of("diamond", 0.2)
+    m2 = mesh_of("naca0012", 0.2)
+    specs, inits = [], []
+    for mesh, mach, aoa, case in [(m1, 2.0, 3.0, "diamond"), (m1, 1.5, -2.0, "diamond"),
+                                  (m2, 0.8, 1.0, "naca0012"), (m1, 3.0, 0.0, "diamond")]:
+        cfg = base_cfg(Mach=mach, aoa=aoa, case=case)
+        W, inlet = M.initialize(mesh, cfg)
+        specs.append(M.case_spec(mesh, inlet, cfg))
+        inits.append(W)
+    kwargs = dict(scheme="SRK2", threshold=0.0, check_every=5, patience=1, max_steps=40)
+    bs = BatchSolver(specs, **kwargs)
+    for i, W in enumerate(inits):
+        bs.set_state(i, W)
+    bs.prepare()
+    bs.run()
+    batched = [bs.get_state(i).cpu().numpy() for i in range(len(specs))]
+    for i, (sp, W) in enumerate(zip(specs, inits)):
+        one = BatchSolver([sp], **kwargs)
+        one.set_state(0, W)
+        one.prepare()
+        one.run()
+        single = one.get_state(0).cpu().numpy()
+        print(f"case {i}: batched vs single", flush=True)
+        assert np.array_equal(batched[i].view(np.uint32), single.view(np.uint32))
