@@ -44,7 +44,7 @@ class Case(C.Structure):
         ("mesh", C.c_void_p), ("opts", Opts), ("cfl", C.c_float), ("tf", C.c_float),
         ("q_ref", C.c_double), ("track_engineering", C.c_int32),
         ("target_cd", C.c_double), ("target_cl", C.c_double), ("cd_tol_abs", C.c_double),
-        ("cl_tol_abs", C.c_double), ("engineering_patience", C.c_int32),
+        ("cl_tol_abs", C.c_double), ("engineering_patience", C.c_int32), ("threshold", C.c_double),
     ]
 
 
@@ -86,7 +86,7 @@ SIGNATURES = {
     "esr_solver_set_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "esr_solver_get_state": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "esr_solver_prepare": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "esr_solver_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "esr_solver_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "esr_solver_finalize": (C.c_int, [C.c_void_p, C.c_void_p]),
     "esr_solver_report": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CaseReport)]),
     "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

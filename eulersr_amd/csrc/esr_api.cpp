@@ -440,7 +440,7 @@ void solver_set_loops(esr_solver* s, hipStream_t st) {
     l.ref_step = 0;
     l.stopping_step = s->N[u];
     l.last_rel = INFINITY;
-    l.threshold = s->so.threshold;
+    l.threshold = c.threshold > 0 ? c.threshold : s->so.threshold;
     l.patience = std::max(1, s->so.patience);
     l.q_ref = c.q_ref;
     l.track_eng = c.track_engineering ? 1 : 0;
@@ -693,7 +693,7 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream) {
   });
 }
 
-esr_status esr_solver_run(esr_solver* s, int64_t* steps, void* stream) {
+esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, void* stream) {
   return guard([&] {
     if (!s) throw Error(ESR_ERR_VALUE, "null solver");
     if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
@@ -708,6 +708,7 @@ esr_status esr_solver_run(esr_solver* s, int64_t* steps, void* stream) {
     }
     int64_t limit = maxN;
     if (s->so.max_steps > 0) limit = std::min(limit, s->so.max_steps);
+    if (until_step > 0) limit = std::min(limit, until_step);
     const bool eng_on = any_engineering(s);
     const int poll_every = std::max(1, 64 / K);
     int checks = 0;

@@ -88,7 +88,7 @@ def build_run_summary(cfg, mesh, cd, cl, delta_s, wall_time_s, **kw):
         "engineering_target_cd": kw.get("engineering_target_cd"),
         "engineering_target_cl": kw.get("engineering_target_cl"),
         "time_scheme": cfg["time_scheme"], "flux": cfg["flux"], "reconstruction": cfg["reconstruction"],
-        "mach": cfg["Mach"], "mesh_path": cfg["mesh_path"], "aoa": float(cfg.get("aoa", 0.0)),
+        "mach": cfg["Mach"], "mesh_path": cfg.get("mesh_path"), "aoa": float(cfg.get("aoa", 0.0)),
     }
 
 

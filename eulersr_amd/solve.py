@@ -27,6 +27,7 @@ class CaseSpec:
     tf: float = 4.0
     q_ref: float = 0.0             # 0.5 rho U^2 L (0: no forces)
     engineering: tuple | None = None  # (target_cd, target_cl, cd_tol_abs, cl_tol_abs, patience)
+    threshold: float = 0.0         # > 0 overrides the solver's stationarity threshold
     tag: dict = field(default_factory=dict)
 
 
@@ -50,6 +51,7 @@ class BatchSolver:
             arr[i].cfl = float(cs.cfl)
             arr[i].tf = float(cs.tf)
             arr[i].q_ref = float(cs.q_ref)
+            arr[i].threshold = float(cs.threshold)
             if cs.engineering is not None:
                 tcd, tcl, cdt, clt, pat = cs.engineering
                 arr[i].track_engineering = 1
@@ -90,9 +92,9 @@ class BatchSolver:
     def prepare(self):
         L.check(L.lib().esr_solver_prepare(self.handle, stream_ptr()), "esr_solver_prepare")
 
-    def run(self):
+    def run(self, until_step=0):
         s = C.c_int64()
-        L.check(L.lib().esr_solver_run(self.handle, C.byref(s), stream_ptr()), "esr_solver_run")
+        L.check(L.lib().esr_solver_run(self.handle, int(until_step), C.byref(s), stream_ptr()), "esr_solver_run")
         self.steps = s.value
         return self.steps
 

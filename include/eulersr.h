@@ -140,6 +140,9 @@ typedef struct {
   int32_t track_engineering; /* engineering CD/CL criterion (main.py:66-114) */
   double target_cd, target_cl, cd_tol_abs, cl_tol_abs;
   int32_t engineering_patience;
+  double threshold;          /* > 0: per-case stationarity threshold (the
+                                HR-probe x 1.001 rule of eval/convergence_case.py:61-66);
+                                <= 0: esr_solve_opts.threshold */
 } esr_case;
 
 typedef struct {
@@ -174,9 +177,10 @@ esr_status esr_solver_get_state(esr_solver* s, int32_t c, float* W, void* stream
 /* Computes dt per case from the current states (helper.py:8-18 once per run,
  * main.py:121) and N = int(tf/dt)+1. */
 esr_status esr_solver_prepare(esr_solver* s, void* stream);
-/* Advances until every case has stopped or reached N (or max_steps);
- * returns the number of steps taken in *steps. */
-esr_status esr_solver_run(esr_solver* s, int64_t* steps, void* stream);
+/* Advances until every case has stopped or reached N (or max_steps, or the
+ * absolute step `until_step` when > 0); *steps = steps taken since prepare.
+ * Successive calls continue the same loop. */
+esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, void* stream);
 /* One-step probe residual + CD/CL of the current states. */
 esr_status esr_solver_finalize(esr_solver* s, void* stream);
 esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out);

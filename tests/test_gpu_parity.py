@@ -114,10 +114,17 @@ def test_forces_match_oracle():
     W, _ = perturbed_state(mesh, cfg, seed=4)
     U = 2.5 * np.sqrt(1.4)
     cd, cl = helper.get_force_coefficients(W, mesh, 1.0, U, 1.0)
-    rcd, rcl = F.forces(W, F.omesh(mesh, np.float32), 1.0, U, 1.0)
-    print(f"case: forces gpu ({cd}, {cl}) oracle ({rcd}, {rcl})", flush=True)
-    assert abs(cd - rcd) <= 1e-6 * max(abs(rcd), 1e-3)
-    assert abs(cl - rcl) <= 1e-6 * max(abs(rcl), 1e-3)
+    om = F.omesh(mesh, np.float32)
+    rcd, rcl = F.forces(W, om, 1.0, U, 1.0)
+    # rounding bound: 1e-6 of the sum of |P n ds| / (q L) (the wall sum cancels)
+    fids, cells, locs = F.wall_faces(om)
+    P = F.getPrimitive(W.astype(np.float64))[cells, 3]
+    q = 0.5 * U ** 2
+    sx = np.sum(np.abs(P * om.normals[cells, locs, 0] * om.surface[fids])) / q
+    sy = np.sum(np.abs(P * om.normals[cells, locs, 1] * om.surface[fids])) / q
+    print(f"case: forces gpu ({cd}, {cl}) oracle ({rcd}, {rcl}) scales ({sx}, {sy})", flush=True)
+    assert abs(cd - rcd) <= 1e-6 * sx
+    assert abs(cl - rcl) <= 1e-6 * sy
 
 
 def test_100_steps_primitives_within_1e5():
