@@ -41,20 +41,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def flux_bytes(items, cells):
-    """Compulsory HBM bytes of one stage-1 flux/update launch: per work-item
-    the stage state (16 B), the three limited face states (48 B) and the new
-    state (16 B); per group cell the geometry read once for all cases of the
-    group: 3 face words (12 B), 3 normals (24 B), 3 lengths (12 B), area and
-    sponge weight (8 B)."""
-    return items * 80 + cells * 56
+def recon_bytes(items, cells, f_items, faces):
+    """Compulsory HBM bytes of one reconstruction launch: per cell work-item
+    the state (16 B) and three limited face states out (48 B); per group cell
+    the geometry read once for all cases of the group: 3 face words (12 B),
+    3 face offsets (24 B), 3 weighted LSQ offsets (24 B), the 2x2 inverse
+    (16 B)."""
+    return items * 64 + cells * 76
 
 
-def recon_bytes(items, cells):
-    """Compulsory bytes of one reconstruction launch: state in (16 B) and three
-    face states out (48 B) per work-item; face words (12 B), face offsets
-    (24 B) and barycentre (8 B) per group cell."""
-    return items * 64 + cells * 44
+def face_bytes(items, cells, f_items, faces):
+    """Compulsory bytes of one face-flux launch: every limited face state read
+    once (48 B per cell work-item), one flux out per face work-item (16 B);
+    per group face the owner word, partner word, normal and length (20 B)."""
+    return items * 48 + f_items * 16 + faces * 20
+
+
+def update_bytes(items, cells, f_items, faces):
+    """Compulsory bytes of one gather + stage-1 update launch: every face flux
+    read once (16 B per face work-item), state in and out (32 B per cell
+    work-item); per group cell 3 face ids, 1/area and sponge weight (20 B)."""
+    return f_items * 16 + items * 32 + cells * 20
 
 
 REF_TF = {"diamond": 5.0}       # eval/live_fvm.py:21-22
@@ -119,6 +126,49 @@ def solve_once(specs, inits, max_steps):
     return bs, t, reps
 
 
+def scale_measurement(meshes, n_cases, steps=200, reps=20):
+    """Kernel throughput at batch scale: n_cases (M, alpha) cases drawn from
+    the M in [0.7, 3.0] x alpha in [-5, 5] grid with default_rng(0), split
+    over the two HR meshes, HLLC + MUSCL; times `steps` SSP-RK2 steps and the
+    stage kernels with HIP events."""
+    import torch
+    from eulersr_amd import main as M
+    from eulersr_amd.solve import BatchSolver
+    rng = np.random.default_rng(0)
+    specs, inits = [], []
+    geoms = list(meshes)
+    for i in range(n_cases):
+        mach = 0.70 + 0.01 * int(rng.integers(0, 231))
+        aoa = -5.0 + 0.5 * int(rng.integers(0, 21))
+        g = geoms[i % len(geoms)]
+        cfg = dict(rho_inf=1.0, p_inf=1.0, gamma=1.4, Mach=mach, aoa=aoa, case=g, time_scheme="SRK2",
+                   flux="HLLC", reconstruction="MUSCL", CFL=0.6, tf=1.0e4)
+        W, inlet = M.initialize(meshes[g], cfg)
+        specs.append(M.case_spec(meshes[g], inlet, cfg))
+        inits.append(W)
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=CHECK_EVERY, patience=PATIENCE)
+    for i, W in enumerate(inits):
+        bs.set_state(i, W)
+    bs.prepare()
+    bs.run(10)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bs.run(10 + steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    sizes = bs.info()
+    ms = bs.time_kernels(reps)
+    kern = {}
+    for name, fn, tk in (("k_recon", recon_bytes, ms[0]), ("k_face<MUSCL,HLLC>", face_bytes, ms[1]),
+                         ("k_update<stage1>", update_bytes, ms[2])):
+        b = fn(*sizes)
+        kern[name] = dict(us_per_launch=tk * 1e3, bytes_per_launch=b, gbs=b / (tk * 1e-3) / 1e9)
+    bs.close()
+    cells = sizes[0]
+    return dict(cases=n_cases, cell_items=cells, us_per_step=t / steps * 1e6,
+                cell_updates_per_s=cells * steps / t, stage_kernels=kern)
+
+
 def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
     """numpy oracle (float32) SSP-RK2 steps on one HR mesh, 1 process."""
     from oracle import fvm as F
@@ -152,6 +202,8 @@ def main():
     ap.add_argument("--max-steps", type=int, default=100000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--scale-cases", type=int, default=256,
+                    help="cases of the at-scale kernel measurement (0: skip)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: reference and timed solves capped at 2000 steps, "
                          "no single-case pass, no CPU baseline")
@@ -201,16 +253,25 @@ def main():
     cell_updates = sum(c * s for c, s in zip(cells, steps_case))
     cups = cell_updates * args.steps / T
 
-    # dominant kernel: live HIP-event timing on the solver's stream
-    items, gcells = last.info()
-    ms_recon, ms_flux = last.time_kernels(args.kernel_reps)
-    fb, rb = flux_bytes(items, gcells), recon_bytes(items, gcells)
-    flux_gbs = fb / (ms_flux * 1e-3) / 1e9
-    recon_gbs = rb / (ms_recon * 1e-3) / 1e9 if ms_recon > 0 else 0.0
-    log(f"k_flux {ms_flux * 1e3:.1f} us/launch ({flux_gbs:.0f} GB/s), "
-        f"k_recon {ms_recon * 1e3:.1f} us/launch ({recon_gbs:.0f} GB/s), items {items}")
+    # stage kernels: live HIP-event timing on the solver's stream
+    sizes = last.info()
+    ms = last.time_kernels(args.kernel_reps)
+    kern = {}
+    for name, fn, t in (("k_recon", recon_bytes, ms[0]), ("k_face<MUSCL,HLLC>", face_bytes, ms[1]),
+                        ("k_update<stage1>", update_bytes, ms[2])):
+        b = fn(*sizes)
+        kern[name] = dict(us_per_launch=t * 1e3, bytes_per_launch=b, gbs=b / (t * 1e-3) / 1e9)
+        log(f"{name}: {t * 1e3:.1f} us/launch, {b / 1e6:.2f} MB -> {kern[name]['gbs']:.0f} GB/s")
+    dom = max(kern, key=lambda k: kern[k]["us_per_launch"])
 
     dcd = [abs(r["cd"] - q["cd"]) for r, q in zip(reps, ref)]
+    scale = None
+    if args.scale_cases > 0 and not args.profile:
+        log(f"at-scale kernel measurement: {args.scale_cases} cases")
+        scale = scale_measurement(meshes, args.scale_cases)
+        for k, v in scale["stage_kernels"].items():
+            log(f"  {k}: {v['us_per_launch']:.1f} us/launch -> {v['gbs']:.0f} GB/s")
+        log(f"  {scale['us_per_step']:.1f} us/step, {scale['cell_updates_per_s']:.3g} cell-updates/s")
     # plain single-case GPU solver: the same cases one at a time
     single_t, single_conv = 0.0, 0
     for sp, W in zip(specs, inits) if not args.profile else []:
@@ -231,10 +292,13 @@ def main():
                                "M in {0.9,1.1,2.0}, aoa 2deg, HLLC+MUSCL, SSP-RK2, CFL 0.6 (0.45 transonic), "
                                "cold start to the HR-probe x1.001 threshold (check 5, patience 3, cap 100k steps)",
                    "cases": len(specs), "cells": cells, "h": args.h},
-        "roofline": {"bound": "hbm", "achieved": flux_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": flux_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_flux<MUSCL,HLLC,stage1>", "bytes_per_launch": fb,
-                     "us_per_launch": ms_flux * 1e3},
+        "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": dom, "bytes_per_launch": kern[dom]["bytes_per_launch"],
+                     "us_per_launch": kern[dom]["us_per_launch"]},
+        "stage_kernels": kern,
+        "at_scale": scale,
+        "launch_sizes": dict(zip(("cell_items", "group_cells", "face_items", "group_faces"), sizes)),
         "cell_updates_per_s": cups,
         "steps_per_case": steps_case,
         "converged": [bool(r["converged"]) for r in reps],
@@ -242,7 +306,6 @@ def main():
         "dcd_vs_reference_field": dcd,
         "thresholds": [sp.threshold for sp in specs],
         "single_case_plain_cases_per_hour": single_rate,
-        "recon_kernel": {"us_per_launch": ms_recon * 1e3, "GB/s": recon_gbs, "bytes_per_launch": rb},
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (numpy oracle)")

@@ -51,7 +51,7 @@ class Case(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [
         ("scheme", C.c_int32), ("threshold", C.c_double), ("check_every", C.c_int32),
-        ("patience", C.c_int32), ("max_steps", C.c_int64), ("local_dt", C.c_int32),
+        ("patience", C.c_int32), ("max_steps", C.c_int64), ("local_dt", C.c_int32), ("no_graph", C.c_int32),
     ]
 
 
@@ -89,9 +89,9 @@ SIGNATURES = {
     "esr_solver_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "esr_solver_finalize": (C.c_int, [C.c_void_p, C.c_void_p]),
     "esr_solver_report": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CaseReport)]),
-    "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
-    "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
-                                          C.POINTER(C.c_double), C.c_void_p]),
+    "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
     "esr_last_error": (C.c_char_p, []),
     "esr_version": (C.c_int32, []),
 }

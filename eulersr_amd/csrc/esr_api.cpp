@@ -37,6 +37,8 @@ Phys make_phys(double gamma, double M) {
   p.c_ir2 = (float)((gamma - 1.0) / (2.0 * gamma));
   p.one_m_g = (float)(1.0 - gamma);
   p.k_exp = (float)(1.0 / (1.0 - gamma));
+  p.inv_gm1M2 = (float)(1.0 / ((gamma - 1.0) * (M * M)));
+  p.inv_gm1 = (float)(1.0 / (gamma - 1.0));
   return p;
 }
 
@@ -99,12 +101,12 @@ struct Engine {
   DevBuf groups_d, cases_d, loops_d;
   struct Cls {
     LaunchSet ls;
-    DevBuf blk;
+    DevBuf blk, fblk;
   };
   std::vector<std::unique_ptr<Cls>> classes;
   bool any_muscl = false, any_lf = false;
-  int64_t elems = 0, rec_elems = 0;
-  DevBuf W0, W1, W2, acc, Wref, rec;
+  int64_t elems = 0, rec_elems = 0, fx_elems = 0;
+  DevBuf W0, W1, W2, acc, Wref, rec, fx;
   int32_t rstride = 1;
   DevBuf part_d, part_r, part_f, nranges, cfl_d, dt_d, forces_d, probe_d, count_d;
   std::vector<std::unique_ptr<DevBuf>> custom_sponge;
@@ -141,7 +143,7 @@ struct Engine {
     // global case order: group by group
     std::vector<int32_t> user_of_global;
     int32_t case0 = 0;
-    int64_t st = 0, rc = 0;
+    int64_t st = 0, rc = 0, fo = 0;
     for (const Key& k : order) {
       const auto& members = gm[k];
       const esr_mesh* m = std::get<0>(k);
@@ -162,6 +164,8 @@ struct Engine {
         g.sponge_g = m->dev.sponge_g;
       }
       st += (int64_t)m->n * g.C;
+      g.fx_off = fo;
+      fo += (int64_t)m->dev.nf * g.C;
       if (std::get<2>(k) == ESR_RECON_MUSCL) {
         g.rec_off = rc;
         rc += 3 * (int64_t)m->n * g.C;
@@ -181,6 +185,7 @@ struct Engine {
     }
     elems = st;
     rec_elems = rc;
+    fx_elems = fo;
     // launch classes
     std::map<std::tuple<int, int, int>, std::vector<int32_t>> cls;
     for (size_t gi = 0; gi < order.size(); ++gi) {
@@ -189,17 +194,24 @@ struct Engine {
     }
     for (auto& kv : cls) {
       auto c = std::make_unique<Cls>();
-      std::vector<int32_t> blk;
+      std::vector<int32_t> blk, fblk;
       for (int32_t gi : kv.second) {
         GroupDev& g = groups[gi];
         const int64_t items = (int64_t)g.m.n * g.C;
         g.blk0 = (int32_t)blk.size();
         g.nblk = (int32_t)((items + BLOCK - 1) / BLOCK);
         for (int32_t b = 0; b < g.nblk; ++b) blk.push_back(gi);
+        const int64_t fitems = (int64_t)g.m.nf * g.C;
+        g.fblk0 = (int32_t)fblk.size();
+        g.fnblk = (int32_t)((fitems + BLOCK - 1) / BLOCK);
+        for (int32_t b = 0; b < g.fnblk; ++b) fblk.push_back(gi);
       }
       upload(c->blk, blk);
+      upload(c->fblk, fblk);
       c->ls.blk_group = c->blk.as<int32_t>();
       c->ls.nblocks = (int32_t)blk.size();
+      c->ls.fblk_group = c->fblk.as<int32_t>();
+      c->ls.fnblocks = (int32_t)fblk.size();
       c->ls.flux = std::get<0>(kv.first);
       c->ls.recon = std::get<1>(kv.first);
       c->ls.entropy = std::get<2>(kv.first);
@@ -233,6 +245,7 @@ struct Engine {
       acc.alloc(elems * sizeof(float4));
     }
     rec.alloc((rec_elems > 0 ? rec_elems : 1) * sizeof(float4));
+    fx.alloc((fx_elems > 0 ? fx_elems : 1) * sizeof(float4));
     // reductions
     rstride = 1;
     std::vector<int32_t> nr(n_cases);
@@ -259,37 +272,30 @@ struct Engine {
   CaseDev* cases() { return cases_d.as<CaseDev>(); }
   const GroupDev* gdev(int32_t gi) const { return groups_d.as<GroupDev>() + gi; }
 
+  // one stage: reconstruction, face fluxes, gather + update into `out`
+  void stage(int mode, const float4* Ws, float4* out, hipStream_t st) {
+    for (auto& c : classes)
+      if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), Ws, rec.as<float4>(), st);
+    for (auto& c : classes) {
+      if (c->ls.flux == ESR_FLUX_LF) {
+        launch_zero_lf(n_cases, cases(), st);
+        launch_lf_cmax(c->ls, cases(), Ws, rec.as<float4>(), st);
+      }
+      launch_face(c->ls, cases(), Ws, rec.as<float4>(), fx.as<float4>(), st);
+    }
+    for (auto& c : classes)
+      launch_update(c->ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
+  }
+
   // one Runge-Kutta step of every active case
   void step(hipStream_t st) {
-    for (const Stage& sg : stages_of(scheme)) {
-      const float4* Ws = buf(sg.ws);
-      for (auto& c : classes)
-        if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), Ws, rec.as<float4>(), st);
-      for (auto& c : classes) {
-        if (c->ls.flux == ESR_FLUX_LF) {
-          launch_zero_lf(n_cases, cases(), st);
-          launch_lf_cmax(c->ls, cases(), Ws, rec.as<float4>(), st);
-        }
-        launch_flux(c->ls, sg.mode, cases(), Ws, W0.as<float4>(), rec.as<float4>(), buf(sg.out), acc.as<float4>(), st);
-      }
-    }
+    for (const Stage& sg : stages_of(scheme)) stage(sg.mode, buf(sg.ws), buf(sg.out), st);
     if (scheme == ESR_SCHEME_EE)
       for (auto& c : classes) launch_commit(c->ls, cases(), W1.as<float4>(), W0.as<float4>(), st);
   }
 
   // residual of W0 into W1 (one case engines)
-  void residual(hipStream_t st) {
-    for (auto& c : classes)
-      if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), W0.as<float4>(), rec.as<float4>(), st);
-    for (auto& c : classes) {
-      if (c->ls.flux == ESR_FLUX_LF) {
-        launch_zero_lf(n_cases, cases(), st);
-        launch_lf_cmax(c->ls, cases(), W0.as<float4>(), rec.as<float4>(), st);
-      }
-      launch_flux(c->ls, MODE_RESID, cases(), W0.as<float4>(), W0.as<float4>(), rec.as<float4>(), W1.as<float4>(),
-                  acc.as<float4>(), st);
-    }
-  }
+  void residual(hipStream_t st) { stage(MODE_RESID, W0.as<float4>(), W1.as<float4>(), st); }
 
   void gather_in(int32_t user_case, const float* W, float4* slab, hipStream_t st) {
     const int32_t gc = global_of_user.at(user_case);
@@ -424,6 +430,15 @@ struct esr_solver {
   int64_t step = 0;
   bool prepared = false;
   bool finalized = false;
+  // solve-loop stream (default priority) and the captured chunk of
+  // check_every steps + one stationarity check
+  hipStream_t gs = nullptr;
+  hipGraphExec_t chunk = nullptr;
+  DevBuf dstep;  // int64[2]: step before / after the chunk
+  ~esr_solver() {
+    if (chunk) (void)hipGraphExecDestroy(chunk);
+    if (gs) (void)hipStreamDestroy(gs);
+  }
 };
 
 namespace {
@@ -693,12 +708,52 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream) {
   });
 }
 
-esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// one stationarity check after step k (host step, or the device counter)
+void enqueue_check(esr_solver* s, int64_t k, bool eng_on, hipStream_t st, int64_t* dstep) {
+  Engine& e = s->eng;
+  const int32_t K = s->so.check_every;
+  launch_mark_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), k, K, 0, st, dstep);
+  e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st);
+  if (eng_on) e.forces(e.W0.as<float4>(), st);
+  launch_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), e.part_d.as<double>(), e.part_r.as<double>(),
+               e.nranges.as<int32_t>(), e.rstride, eng_on ? e.forces_d.as<double>() : nullptr, k, K, st, dstep);
+}
+
+void capture_chunk(esr_solver* s, bool eng_on) {
+  Engine& e = s->eng;
+  const int32_t K = s->so.check_every;
+  hipGraph_t g = nullptr;
+  hip_check(hipStreamBeginCapture(s->gs, hipStreamCaptureModeThreadLocal), "begin capture");
+  try {
+    for (int32_t q = 0; q < K; ++q) e.step(s->gs);
+    enqueue_check(s, 0, eng_on, s->gs, s->dstep.as<int64_t>());
+  } catch (...) {
+    (void)hipStreamEndCapture(s->gs, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(s->gs, &g), "end capture");
+  hipError_t err = hipGraphInstantiate(&s->chunk, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  hip_check(err, "graph instantiate");
+}
+
+}  // namespace
+
+extern "C" esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, void* stream) {
   return guard([&] {
     if (!s) throw Error(ESR_ERR_VALUE, "null solver");
     if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
-    hipStream_t st = S(stream);
     Engine& e = s->eng;
+    // everything before this call on the caller's stream (states, prepare)
+    hip_check(hipStreamSynchronize(S(stream)), "caller stream sync");
+    if (!s->gs) hip_check(hipStreamCreateWithFlags(&s->gs, hipStreamNonBlocking), "stream create");
+    if (!s->dstep.p) s->dstep.alloc(2 * sizeof(int64_t));
+    hipStream_t st = s->gs;
     const int32_t K = s->so.check_every;
     int64_t maxN = 0;
     std::set<int64_t> Ns;
@@ -710,30 +765,43 @@ esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, voi
     if (s->so.max_steps > 0) limit = std::min(limit, s->so.max_steps);
     if (until_step > 0) limit = std::min(limit, until_step);
     const bool eng_on = any_engineering(s);
+    const bool use_graph = !s->so.no_graph;
     const int poll_every = std::max(1, 64 / K);
     int checks = 0;
-    while (s->step < limit) {
-      s->step += 1;
-      e.step(st);
-      const int64_t k = s->step;
-      if (k % K == 0 || Ns.count(k)) {
-        launch_mark_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), k, K, 0, st);
-        e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st);
-        if (eng_on) e.forces(e.W0.as<float4>(), st);
-        launch_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), e.part_d.as<double>(), e.part_r.as<double>(),
-                     e.nranges.as<int32_t>(), e.rstride, eng_on ? e.forces_d.as<double>() : nullptr, k, K, st);
-        if (++checks % poll_every == 0) {
-          launch_count_active(e.n_cases, e.cases(), e.count_d.as<int32_t>(), st);
-          hip_check(hipMemcpyAsync(e.count_h, e.count_d.p, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count");
-          hip_check(hipStreamSynchronize(st), "poll sync");
-          if (*e.count_h == 0) break;
+    bool all_done = false;
+    while (s->step < limit && !all_done) {
+      const int64_t k0 = s->step;
+      auto nextN = Ns.upper_bound(k0);
+      const bool n_inside = nextN != Ns.end() && *nextN < k0 + K;
+      bool checked = false;
+      if (use_graph && k0 % K == 0 && k0 + K <= limit && !n_inside) {
+        if (!s->chunk) capture_chunk(s, eng_on);
+        launch_set_step(s->dstep.as<int64_t>(), k0, st);
+        hip_check(hipGraphLaunch(s->chunk, st), "graph launch");
+        s->step = k0 + K;
+        checked = true;
+      } else {
+        s->step = k0 + 1;
+        e.step(st);
+        const int64_t k = s->step;
+        if (k % K == 0 || Ns.count(k)) {
+          enqueue_check(s, k, eng_on, st, nullptr);
+          checked = true;
         }
+      }
+      if (checked && ++checks % poll_every == 0) {
+        launch_count_active(e.n_cases, e.cases(), e.count_d.as<int32_t>(), st);
+        hip_check(hipMemcpyAsync(e.count_h, e.count_d.p, sizeof(int32_t), hipMemcpyDeviceToHost, st), "count");
+        hip_check(hipStreamSynchronize(st), "poll sync");
+        all_done = (*e.count_h == 0);
       }
     }
     hip_check(hipStreamSynchronize(st), "run sync");
     if (steps) *steps = s->step;
   });
 }
+
+extern "C" {
 
 esr_status esr_solver_finalize(esr_solver* s, void* stream) {
   return guard([&] {
@@ -793,57 +861,64 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
   });
 }
 
-esr_status esr_solver_info(const esr_solver* s, int64_t* work_items, int64_t* group_cells) {
+esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* group_cells, int64_t* face_items,
+                           int64_t* group_faces) {
   return guard([&] {
     if (!s) throw Error(ESR_ERR_VALUE, "null solver");
-    int64_t items = 0, cells = 0;
+    int64_t items = 0, cells = 0, fitems = 0, faces = 0;
     for (const GroupDev& g : s->eng.groups) {
       items += (int64_t)g.m.n * g.C;
       cells += g.m.n;
+      fitems += (int64_t)g.m.nf * g.C;
+      faces += g.m.nf;
     }
-    if (work_items) *work_items = items;
+    if (cell_items) *cell_items = items;
     if (group_cells) *group_cells = cells;
+    if (face_items) *face_items = fitems;
+    if (group_faces) *group_faces = faces;
   });
 }
 
-esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms_recon, double* ms_flux, void* stream) {
+esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream) {
   return guard([&] {
-    if (!s || reps <= 0) throw Error(ESR_ERR_VALUE, "bad arguments");
+    if (!s || reps <= 0 || !ms) throw Error(ESR_ERR_VALUE, "bad arguments");
     hipStream_t st = S(stream);
     Engine& e = s->eng;
     launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
-    hipEvent_t a, b, c;
-    hip_check(hipEventCreate(&a), "event");
-    hip_check(hipEventCreate(&b), "event");
-    hip_check(hipEventCreate(&c), "event");
-    // warm the code objects once
-    for (auto& cl : e.classes)
-      if (cl->ls.recon == ESR_RECON_MUSCL) launch_recon(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st);
-    for (auto& cl : e.classes)
-      launch_flux(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
-                  e.W1.as<float4>(), e.acc.as<float4>(), st);
-    hip_check(hipEventRecord(a, st), "record");
-    for (int32_t r = 0; r < reps; ++r)
+    hipEvent_t ev[4];
+    for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
+    auto recon = [&] {
       for (auto& cl : e.classes)
         if (cl->ls.recon == ESR_RECON_MUSCL)
           launch_recon(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st);
-    hip_check(hipEventRecord(b, st), "record");
-    for (int32_t r = 0; r < reps; ++r)
+    };
+    auto face = [&] {
+      for (auto& cl : e.classes) launch_face(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(),
+                                             e.fx.as<float4>(), st);
+    };
+    auto update = [&] {
       for (auto& cl : e.classes)
-        launch_flux(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
-                    e.W1.as<float4>(), e.acc.as<float4>(), st);
-    hip_check(hipEventRecord(c, st), "record");
-    hip_check(hipEventSynchronize(c), "event sync");
-    float t1 = 0.f, t2 = 0.f;
-    hip_check(hipEventElapsedTime(&t1, a, b), "elapsed");
-    hip_check(hipEventElapsedTime(&t2, b, c), "elapsed");
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipEventDestroy(c);
+        launch_update(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.fx.as<float4>(),
+                      e.W1.as<float4>(), e.acc.as<float4>(), st);
+    };
+    recon();
+    face();
+    update();  // code objects loaded, buffers valid
+    hip_check(hipEventRecord(ev[0], st), "record");
+    for (int32_t r = 0; r < reps; ++r) recon();
+    hip_check(hipEventRecord(ev[1], st), "record");
+    for (int32_t r = 0; r < reps; ++r) face();
+    hip_check(hipEventRecord(ev[2], st), "record");
+    for (int32_t r = 0; r < reps; ++r) update();
+    hip_check(hipEventRecord(ev[3], st), "record");
+    hip_check(hipEventSynchronize(ev[3]), "event sync");
+    for (int q = 0; q < 3; ++q) {
+      float t = 0.f;
+      hip_check(hipEventElapsedTime(&t, ev[q], ev[q + 1]), "elapsed");
+      ms[q] = (double)t / reps;
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
     if (s->prepared) launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
-    const double ncls = (double)e.classes.size();
-    if (ms_recon) *ms_recon = e.any_muscl ? (double)t1 / reps : 0.0;
-    if (ms_flux) *ms_flux = (double)t2 / reps / (ncls > 0 ? 1.0 : 1.0);
   });
 }
 

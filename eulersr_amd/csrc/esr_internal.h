@@ -33,22 +33,42 @@ enum Ghost : uint32_t { G_WALL = 0, G_INLET = 1, G_OUTLET = 2, G_SUBSONIC = 3, G
 __host__ __device__ inline bool fref_is_bnd(uint32_t f) { return (f & FREF_BND) == FREF_BND; }
 
 // Device view of one renumbered mesh (structure of arrays).
+//
+// Cell arrays are indexed by the renumbered cell, [k][n] arrays by local
+// face k then cell. Faces are "logical faces": one per interior face pair,
+// one per periodic face pair, one per boundary face; each is owned by the
+// first cell (in renumbered order) that touches it and carries the owner's
+// outward normal. A cell reads the flux of its local face k from
+// cface[k][i] (bit 31 set: the cell is the partner and takes -F).
 struct MeshDev {
   int32_t n;             // cells
+  int32_t nf;            // logical faces
   int32_t n_wall;        // force-marker faces
-  const uint32_t* fref;  // [3][n]
-  const float2* bary;    // [n]
-  const float2* dxm;     // [3][n]  face midpoint - barycentre (float32, reference order)
-  const float2* nrm;     // [3][n]
-  const float* len;      // [3][n]
-  const float* area;     // [n]
-  const float* sponge_g; // [n]    blend^3 for the default sponge width
+  // reconstruction (per cell)
+  const uint32_t* fref;  // [3][n] neighbour word (layout above)
+  const float2* dxm;     // [3][n] face midpoint - barycentre (float32, reference order)
+  const float2* wdl;     // [3][n] LSQ offset scaled by its weight, w_k * dx_k
+  const float4* ainv;    // [n]    inverse of the LSQ normal matrix (a00, a01, a10, a11)
+  const float2* nrm;     // [3][n] outward normals (wall ghosts)
+  // flux (per logical face)
+  const uint32_t* f_own;  // [nf] (slot << 30) | owner cell
+  const uint32_t* f_ref;  // [nf] partner word of the owner's side (fref layout)
+  const float2* f_nrm;    // [nf] owner's outward normal
+  const float* f_len;     // [nf] face length
+  // update (per cell)
+  const uint32_t* cface;  // [3][n] logical face id | sign bit
+  const float* inv_area;  // [n]
+  // time step (helper.get_dt)
+  const float* len;       // [3][n] face lengths seen from the cell
+  const float* area;      // [n]
+  const float* sponge_g;  // [n] blend^3 for the default sponge width
   const int32_t* wall_cell;  // [n_wall]
   const float* wall_nx;      // [n_wall]
   const float* wall_ny;      // [n_wall]
   const float* wall_ds;      // [n_wall]
   float4 inlet_subsonic;     // primitives (helper.py:181)
 };
+constexpr uint32_t CFACE_NEG = 1u << 31;
 
 // Physical constants of a group, derived in double precision on the host and
 // rounded to float32 once (as Python floats meet float32 arrays in the
@@ -65,6 +85,8 @@ struct Phys {
   float c_ir2;      // (gamma - 1) / (2 gamma)
   float one_m_g;    // 1 - gamma
   float k_exp;      // 1 / (1 - gamma)
+  float inv_gm1M2;  // 1 / ((gamma - 1) M**2)
+  float inv_gm1;    // 1 / (gamma - 1)
 };
 
 // A group: C cases sharing one mesh and one kernel configuration. The state
@@ -75,8 +97,11 @@ struct GroupDev {
   int32_t case0;         // global index of the group's first case
   int64_t st_off;        // float4 offset of the group's state slab
   int64_t rec_off;       // float4 offset of the group's reconstruction slab ([3][n][C])
-  int32_t blk0;          // first block of the group inside its launch
-  int32_t nblk;          // blocks of the group
+  int64_t fx_off;        // float4 offset of the group's face-flux slab ([nf][C])
+  int32_t blk0;          // first cell block of the group inside its launch
+  int32_t nblk;          // cell blocks of the group
+  int32_t fblk0;         // first face block of the group inside its launch
+  int32_t fnblk;         // face blocks of the group
   Phys ph;
   int32_t sponge;
   const float* sponge_g; // mesh default or a custom width
@@ -184,7 +209,8 @@ struct esr_mesh {
   float y_min = 0.f, y_max = 0.f;
   int32_t sponge = 1;
   esr::MeshDev dev{};
-  esr::DevBuf b_fref, b_bary, b_dxm, b_nrm, b_len, b_area, b_sponge, b_perm;
+  esr::DevBuf b_fref, b_dxm, b_wdl, b_ainv, b_nrm, b_fown, b_fref2, b_fnrm, b_flen, b_cface, b_iarea, b_sponge,
+      b_perm, b_len, b_area;
   esr::DevBuf b_wcell, b_wnx, b_wny, b_wds;
   int64_t device_bytes = 0;
   struct Workspace* ws = nullptr;  // single-state scratch (esr_kernels.hip)

@@ -8,15 +8,19 @@
 // integer max on non-negative float bits, which is order independent).
 //
 // Per stage of a Runge-Kutta step:
-//   k_recon  (MUSCL only): primitives, boundary ghosts, weighted LSQ
-//            gradient, Venkatakrishnan limiter and the three limited face
+//   k_recon  (MUSCL only, per cell): primitives, boundary ghosts, weighted
+//            LSQ gradient, Venkatakrishnan limiter and the three limited face
 //            states of each cell (helper.py:136-153, 233-256;
 //            euler_solver.py:36-122).
-//   k_flux   : per-side numerical flux of the three faces (the reference
-//            evaluates every interior face once from each side,
-//            euler_solver.py:107-109), face-length weighted sum, division by
-//            the area, sponge source and the stage update
-//            (euler_solver.py:141-143, 376-420, 602-667; helper.py:262-295).
+//   k_face   (per logical face): numerical flux times face length, once per
+//            face in the owner's orientation. The reference evaluates every
+//            interior face once from each side (euler_solver.py:107-109);
+//            the partner's flux is the exact negation here, which equals the
+//            reference up to rounding (F(L,R,n) = -F(R,L,-n)).
+//   k_update (per cell): deterministic gather of the three face fluxes
+//            (no atomics), division by the area, sponge source and the
+//            Runge-Kutta stage update (euler_solver.py:141-143, 602-667;
+//            helper.py:262-295).
 // Reductions (FP64 norms, FP64 CD/CL wall sums, dt minimum) use fixed-order
 // wave/LDS trees and per-range partials, so every run is bit-reproducible.
 #include <hip/hip_runtime.h>
@@ -26,6 +30,16 @@
 #include "esr_launch.h"
 
 namespace esr {
+
+// ---------------------------------------------------------------------------
+// arithmetic helpers
+// ---------------------------------------------------------------------------
+
+// a / b through the hardware reciprocal (1 ulp) instead of the IEEE division
+// sequence: the kernels are issue-bound and the reference tolerance is far
+// above the difference.
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
 
 // ---------------------------------------------------------------------------
 // state conversions (helper.py:25-46)
@@ -38,16 +52,17 @@ struct Prim {
 // getPrimitive: P = (E - 0.5 rho (u^2+v^2)) * M^2 * (gamma-1)
 __device__ __forceinline__ Prim prim_of(const float4 W, float M2, float gm1) {
   Prim q;
+  const float ir = frcp(W.x);
   q.r = W.x;
-  q.u = W.y / W.x;
-  q.v = W.z / W.x;
+  q.u = W.y * ir;
+  q.v = W.z * ir;
   q.P = (W.w - 0.5f * q.r * (q.u * q.u + q.v * q.v)) * M2 * gm1;
   return q;
 }
 
 // getConserved: E = P / ((gamma-1) M^2) + 0.5 rho (u^2+v^2)
-__device__ __forceinline__ float4 cons_of(const Prim q, float gm1M2) {
-  return make_float4(q.r, q.r * q.u, q.r * q.v, q.P / gm1M2 + 0.5f * q.r * (q.u * q.u + q.v * q.v));
+__device__ __forceinline__ float4 cons_of(const Prim q, float inv_gm1M2) {
+  return make_float4(q.r, q.r * q.u, q.r * q.v, q.P * inv_gm1M2 + 0.5f * q.r * (q.u * q.u + q.v * q.v));
 }
 
 __device__ __forceinline__ Prim as_prim(const float4 p) { return Prim{p.x, p.y, p.z, p.w}; }
@@ -58,17 +73,17 @@ __device__ __forceinline__ float4 ghost_state(uint32_t kind, const float4 Wi, co
                                               const float4 insub, const float4* __restrict__ Ws, int64_t wrap_idx) {
   if (kind == G_WALL) {
     const Prim p = prim_of(Wi, 1.0f, 0.4f);
-    const float vn = (p.u - 0.0f) * n.x + (p.v - 0.0f) * n.y;
-    const float bx = (p.u - 0.0f) - 2.0f * vn * n.x;
-    const float by = (p.v - 0.0f) - 2.0f * vn * n.y;
-    return cons_of(Prim{p.r, bx + 0.0f, by + 0.0f, p.P}, 0.4f);
+    const float vn = p.u * n.x + p.v * n.y;
+    const float bx = p.u - 2.0f * vn * n.x;
+    const float by = p.v - 2.0f * vn * n.y;
+    return cons_of(Prim{p.r, bx, by, p.P}, 2.5f);
   }
   if (kind == G_INLET) return value;
   if (kind == G_OUTLET) return Wi;
   if (kind == G_SUBSONIC) {
     const Prim p = prim_of(Wi, 1.0f, 0.4f);
     const float r = insub.x, u = insub.y, v = insub.z;
-    return make_float4(r, r * u, r * v, p.P / 0.4f + 0.5f * r * (u * u + v * v));
+    return make_float4(r, r * u, r * v, p.P * 2.5f + 0.5f * r * (u * u + v * v));
   }
   return Ws[wrap_idx];
 }
@@ -79,24 +94,25 @@ __device__ __forceinline__ float4 ghost_state(uint32_t kind, const float4 Wi, co
 
 struct Item {
   const GroupDev* G;
-  int32_t cell;
+  int32_t idx;  // cell (cell kernels) or logical face (face kernels)
   int32_t c;
-  int64_t sidx;
+  int64_t sidx; // state index (cell kernels)
 };
 
+template <bool FACES>
 __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const int32_t* __restrict__ blk_group,
                                          const CaseDev* __restrict__ cases, Item& it) {
   const GroupDev* G = groups + blk_group[blockIdx.x];
   const int32_t C = G->C;
-  const int64_t t = (int64_t)(blockIdx.x - G->blk0) * BLOCK + threadIdx.x;
-  const int32_t cell = (int32_t)(t / C);
-  const int32_t c = (int32_t)(t - (int64_t)cell * C);
-  if (cell >= G->m.n) return false;
+  const int64_t t = (int64_t)(blockIdx.x - (FACES ? G->fblk0 : G->blk0)) * BLOCK + threadIdx.x;
+  const int32_t idx = (int32_t)(t / C);
+  const int32_t c = (int32_t)(t - (int64_t)idx * C);
+  if (idx >= (FACES ? G->m.nf : G->m.n)) return false;
   if (!cases[G->case0 + c].active) return false;
   it.G = G;
-  it.cell = cell;
+  it.idx = idx;
   it.c = c;
-  it.sidx = G->st_off + (int64_t)cell * C + c;
+  it.sidx = FACES ? 0 : G->st_off + (int64_t)idx * C + c;
   return true;
 }
 
@@ -106,6 +122,9 @@ __device__ __forceinline__ int64_t sidx_of(const GroupDev* G, int32_t cell, int3
 __device__ __forceinline__ int64_t ridx_of(const GroupDev* G, int k, int32_t cell, int32_t c) {
   return G->rec_off + ((int64_t)k * G->m.n + cell) * G->C + c;
 }
+__device__ __forceinline__ int64_t fidx_of(const GroupDev* G, int32_t face, int32_t c) {
+  return G->fx_off + (int64_t)face * G->C + c;
+}
 
 // ---------------------------------------------------------------------------
 // reconstruction: LSQ gradient + Venkatakrishnan + MUSCL face states
@@ -113,7 +132,7 @@ __device__ __forceinline__ int64_t ridx_of(const GroupDev* G, int k, int32_t cel
 
 // venkatakrishnan(a, b) with K = 0 (euler_solver.py:36-39)
 __device__ __forceinline__ float venkat(float a, float b) {
-  return (a * a + 2.0f * a * b) / (a * a + 2.0f * (b * b) + a * b + 1e-7f);
+  return fdiv(a * a + 2.0f * a * b, a * a + 2.0f * (b * b) + a * b + 1e-7f);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ groups,
@@ -121,132 +140,85 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
                                                  const CaseDev* __restrict__ cases,
                                                  const float4* __restrict__ Ws, float4* __restrict__ rec) {
   Item it;
-  if (!map_item(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
-  const int32_t n = m.n, cell = it.cell;
+  const int32_t n = m.n, cell = it.idx;
   const CaseDev& cs = cases[G->case0 + it.c];
 
   const float4 Wi = Ws[it.sidx];
   const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
-  const float2 bi = m.bary[cell];
 
   Prim pr[3];
-  float2 dl[3], dm[3];
   bool bnd[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const uint32_t f = m.fref[(int64_t)k * n + cell];
-    dm[k] = m.dxm[(int64_t)k * n + cell];
     float4 WR;
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
       const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + cell] : make_float2(0.f, 0.f);
       WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
-      dl[k] = make_float2(2.0f * dm[k].x, 2.0f * dm[k].y);
       bnd[k] = true;
     } else {
-      const int32_t j = (int32_t)(f & FREF_IDX);
-      WR = Ws[sidx_of(G, j, it.c)];
-      if (f & FREF_PER) {
-        dl[k] = make_float2(2.0f * dm[k].x, 2.0f * dm[k].y);
-      } else {
-        const float2 bj = m.bary[j];
-        dl[k] = make_float2(bj.x - bi.x, bj.y - bi.y);
-      }
+      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
       bnd[k] = false;
     }
     pr[k] = prim_of(WR, ph.M2, ph.gm1);
   }
 
-  // weighted least squares, w = 1 / |dx|^2 (helper.py:144-152)
-  float wx[3], wy[3], wt[3];
+  // weighted least squares (helper.py:136-153): grad = A^-1 sum_k w_k dW_k dl_k
+  const float2 w0 = m.wdl[cell], w1 = m.wdl[(int64_t)n + cell], w2 = m.wdl[2 * (int64_t)n + cell];
+  const float4 ai = m.ainv[cell];
+  float2 dm[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const float nr = sqrtf(dl[k].x * dl[k].x + dl[k].y * dl[k].y);
-    wt[k] = 1.0f / (nr * nr);
-    wx[k] = wt[k] * dl[k].x;
-    wy[k] = wt[k] * dl[k].y;
-  }
-  const float A00 = (wx[0] * dl[0].x + wx[1] * dl[1].x) + wx[2] * dl[2].x;
-  const float A01 = (wx[0] * dl[0].y + wx[1] * dl[1].y) + wx[2] * dl[2].y;
-  const float A10 = (wy[0] * dl[0].x + wy[1] * dl[1].x) + wy[2] * dl[2].x;
-  const float A11 = (wy[0] * dl[0].y + wy[1] * dl[1].y) + wy[2] * dl[2].y;
-  // LU with partial pivoting of the 2x2 system
-  const bool swp = fabsf(A10) > fabsf(A00);
-  const float p00 = swp ? A10 : A00, p01 = swp ? A11 : A01;
-  const float p10 = swp ? A00 : A10, p11 = swp ? A01 : A11;
-  const float l = p10 / p00;
-  const float u11 = p11 - l * p01;
+  for (int k = 0; k < 3; ++k) dm[k] = m.dxm[(int64_t)k * n + cell];
 
   const float qi[4] = {pi.r, pi.u, pi.v, pi.P};
-  float gx[4], gy[4], Wmin[4], Wmax[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    float q[3];
-    q[0] = (v == 0) ? pr[0].r : (v == 1) ? pr[0].u : (v == 2) ? pr[0].v : pr[0].P;
-    q[1] = (v == 0) ? pr[1].r : (v == 1) ? pr[1].u : (v == 2) ? pr[1].v : pr[1].P;
-    q[2] = (v == 0) ? pr[2].r : (v == 1) ? pr[2].u : (v == 2) ? pr[2].v : pr[2].P;
-    float bx = 0.f, by = 0.f;
-    float dW[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) dW[k] = q[k] - qi[v];
-    bx = ((wt[0] * dW[0]) * dl[0].x + (wt[1] * dW[1]) * dl[1].x) + (wt[2] * dW[2]) * dl[2].x;
-    by = ((wt[0] * dW[0]) * dl[0].y + (wt[1] * dW[1]) * dl[1].y) + (wt[2] * dW[2]) * dl[2].y;
-    const float c0 = swp ? by : bx, c1 = swp ? bx : by;
-    const float y1 = c1 - l * c0;
-    const float x1 = y1 / u11;
-    const float x0 = (c0 - p01 * x1) / p00;
-    gx[v] = x0;
-    gy[v] = x1;
-    float mn = qi[v], mx = qi[v];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      mn = fminf(mn, q[k]);
-      mx = fmaxf(mx, q[k]);
-    }
-    Wmin[v] = mn;
-    Wmax[v] = mx;
-  }
-
-  // Venkatakrishnan limiter, min over the three faces (euler_solver.py:45-67)
   float D[3][4];
   float phi[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
-    float ph3[3];
+    float q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q[k] = (v == 0) ? pr[k].r : (v == 1) ? pr[k].u : (v == 2) ? pr[k].v : pr[k].P;
+    const float d0 = q[0] - qi[v], d1 = q[1] - qi[v], d2 = q[2] - qi[v];
+    const float bx = (d0 * w0.x + d1 * w1.x) + d2 * w2.x;
+    const float by = (d0 * w0.y + d1 * w1.y) + d2 * w2.y;
+    const float gx = ai.x * bx + ai.y * by;
+    const float gy = ai.z * bx + ai.w * by;
+    const float mn = fminf(fminf(qi[v], q[0]), fminf(q[1], q[2]));
+    const float mx = fmaxf(fmaxf(qi[v], q[0]), fmaxf(q[1], q[2]));
+    // Venkatakrishnan limiter, min over the three faces (euler_solver.py:45-67)
+    float p3[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      const float d = dm[k].x * gx[v] + dm[k].y * gy[v];
+      const float d = dm[k].x * gx + dm[k].y * gy;
       D[k][v] = d;
       float p = 1.0f;
-      if (d > 1e-7f) p = venkat(Wmax[v] - qi[v], d);
-      if (d < -1e-7f) p = venkat(Wmin[v] - qi[v], d);
-      ph3[k] = p;
+      if (d > 1e-7f) p = venkat(mx - qi[v], d);
+      if (d < -1e-7f) p = venkat(mn - qi[v], d);
+      p3[k] = p;
     }
-    phi[v] = fminf(fminf(ph3[0], ph3[1]), ph3[2]);
+    phi[v] = fminf(fminf(p3[0], p3[1]), p3[2]);
   }
 
-  // limited face states (primitives); no limiting slope on marker > 1 faces
+  // limited face states (primitives); no slope on marker > 1 faces
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     float4 o;
-    const float s0 = bnd[k] ? 0.0f : phi[0];
-    const float s1 = bnd[k] ? 0.0f : phi[1];
-    const float s2 = bnd[k] ? 0.0f : phi[2];
-    const float s3 = bnd[k] ? 0.0f : phi[3];
-    o.x = qi[0] + s0 * D[k][0];
-    o.y = qi[1] + s1 * D[k][1];
-    o.z = qi[2] + s2 * D[k][2];
-    o.w = qi[3] + s3 * D[k][3];
+    o.x = bnd[k] ? qi[0] : qi[0] + phi[0] * D[k][0];
+    o.y = bnd[k] ? qi[1] : qi[1] + phi[1] * D[k][1];
+    o.z = bnd[k] ? qi[2] : qi[2] + phi[2] * D[k][2];
+    o.w = bnd[k] ? qi[3] : qi[3] + phi[3] * D[k][3];
     rec[ridx_of(G, k, cell, it.c)] = o;
   }
 }
 
 // ---------------------------------------------------------------------------
 // numerical fluxes (euler_solver.py:145-591); each returns the total flux
-// F = central - D of one side-face.
+// F = central - D of one face for the owner's normal.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ void central_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R,
@@ -269,6 +241,10 @@ __device__ __forceinline__ void phys_flux(const float4 W, const Prim& q, float u
   F[3] = (W.w + q.P * invM2) * un;
 }
 
+__device__ __forceinline__ float sound(const Prim& q, const Phys& ph) {
+  return __builtin_sqrtf(fabsf(ph.g * q.P * frcp(q.r))) * ph.invM;
+}
+
 __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
                                           float ny, const Phys& ph, float D[4]) {
   const float unL = L.u * nx + L.v * ny;
@@ -276,8 +252,8 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
   const float tx = -ny, ty = nx;
   const float utL = L.u * tx + L.v * ty;
   const float utR = R.u * tx + R.v * ty;
-  const float aL = sqrtf(fabsf(ph.g * L.P / L.r)) * ph.invM;
-  const float aR = sqrtf(fabsf(ph.g * R.P / R.r)) * ph.invM;
+  const float aL = sound(L, ph);
+  const float aR = sound(R, ph);
   float SL = fminf(unL - aL, unR - aR);
   float SR = fmaxf(unL + aL, unR + aR);
   SL = (SL == SR) ? SL - 1e-6f : SL;
@@ -287,7 +263,7 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
   phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
   float den = L.r * (SL - unL) - R.r * (SR - unR);
   den = (den == 0.0f) ? 1e-12f : den;
-  const float Ss = (R.P - L.P + L.r * unL * (SL - unL) - R.r * unR * (SR - unR)) / den;
+  const float Ss = fdiv(R.P - L.P + L.r * unL * (SL - unL) - R.r * unR * (SR - unR), den);
   float F[4];
   if (SL >= 0.0f) {
 #pragma unroll
@@ -296,22 +272,24 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
     const float PsL = L.P + L.r * (SL - unL) * (Ss - unL);
     const float PsR = R.P + R.r * (SR - unR) * (Ss - unR);
     const float Ps = 0.5f * (PsL + PsR);
-    if (SL <= 0.0f && Ss >= 0.0f) {
-      const float rs = L.r * (SL - unL) / (SL - Ss);
-      const float ux = Ss * nx + utL * tx, uy = Ss * ny + utL * ty;
-      const float Es = ((SL - unL) * WL.w - L.P * unL + Ps * Ss) / (SL - Ss);
-      F[0] = FL[0] + SL * (rs - WL.x);
-      F[1] = FL[1] + SL * (rs * ux - WL.y);
-      F[2] = FL[2] + SL * (rs * uy - WL.z);
-      F[3] = FL[3] + SL * (Es - WL.w);
-    } else if (Ss <= 0.0f && SR >= 0.0f) {
-      const float rs = R.r * (SR - unR) / (SR - Ss);
-      const float ux = Ss * nx + utR * tx, uy = Ss * ny + utR * ty;
-      const float Es = ((SR - unR) * WR.w - R.P * unR + Ps * Ss) / (SR - Ss);
-      F[0] = FR[0] + SR * (rs - WR.x);
-      F[1] = FR[1] + SR * (rs * ux - WR.y);
-      F[2] = FR[2] + SR * (rs * uy - WR.z);
-      F[3] = FR[3] + SR * (Es - WR.w);
+    // left or right star state, selected without duplicating the arithmetic
+    const bool left = (SL <= 0.0f && Ss >= 0.0f);
+    const bool right = !left && (Ss <= 0.0f && SR >= 0.0f);
+    if (left || right) {
+      const float S = left ? SL : SR;
+      const Prim& K = left ? L : R;
+      const float4 WK = left ? WL : WR;
+      const float unK = left ? unL : unR;
+      const float utK = left ? utL : utR;
+      const float* FK = left ? FL : FR;
+      const float idn = frcp(S - Ss);
+      const float rs = K.r * (S - unK) * idn;
+      const float ux = Ss * nx + utK * tx, uy = Ss * ny + utK * ty;
+      const float Es = ((S - unK) * WK.w - K.P * unK + Ps * Ss) * idn;
+      F[0] = FK[0] + S * (rs - WK.x);
+      F[1] = FK[1] + S * (rs * ux - WK.y);
+      F[2] = FK[2] + S * (rs * uy - WK.z);
+      F[3] = FK[3] + S * (Es - WK.w);
     } else {
 #pragma unroll
       for (int v = 0; v < 4; ++v) F[v] = FR[v];
@@ -322,54 +300,57 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
 }
 
 __device__ __forceinline__ float wave_speed(const Prim& q, float nx, float ny, const Phys& ph) {
-  return sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM + fabsf(q.u * nx + q.v * ny);
+  return sound(q, ph) + fabsf(q.u * nx + q.v * ny);
 }
 
 // entropy variables (helper.py:48-70)
 __device__ __forceinline__ float4 entropy_vars(const float4 W, const Phys& ph) {
-  const float r = W.x, u = W.y / r, v = W.z / r;
+  const float r = W.x, ir = frcp(r), u = W.y * ir, v = W.z * ir;
   const float P = (W.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
+  const float iP = frcp(P);
   const float s = logf(P / powf(r, ph.g));
   float4 V;
-  V.x = (ph.g - s) / ph.gm1 - 0.5f * r * (u * u + v * v) / P;
-  V.y = W.y / P;
-  V.z = W.z / P;
-  V.w = -r / P;
+  V.x = (ph.g - s) * ph.inv_gm1 - 0.5f * r * (u * u + v * v) * iP;
+  V.y = W.y * iP;
+  V.z = W.z * iP;
+  V.w = -r * iP;
   return V;
 }
 
 // d/dt getConserved_from_Entropy(eta + t deta) at t = 0 (the jax.jvp of
 // euler_solver.py:234-238), chain rule over helper.py:72-86.
 __device__ __forceinline__ void jvp_u_of_v(const float4 e, const float4 d, const Phys& ph, float out[4]) {
-  const float u = -e.y / e.w;
-  const float du = -(d.y / e.w - e.y * d.w / (e.w * e.w));
-  const float v = -e.z / e.w;
-  const float dv = -(d.z / e.w - e.z * d.w / (e.w * e.w));
+  const float ie = frcp(e.w);
+  const float u = -e.y * ie;
+  const float du = -(d.y * ie - e.y * d.w * (ie * ie));
+  const float v = -e.z * ie;
+  const float dv = -(d.z * ie - e.z * d.w * (ie * ie));
   const float q = 0.5f * (u * u + v * v);
   const float dq = u * du + v * dv;
   const float s = ph.g - ph.gm1 * (e.x - q * e.w);
   const float ds = -ph.gm1 * (d.x - (dq * e.w + q * d.w));
   const float a = powf(-e.w, ph.k_exp);
   const float da = ph.k_exp * powf(-e.w, ph.k_exp - 1.0f) * (-d.w);
-  const float b = expf(s / ph.one_m_g);
-  const float db = b * ds / ph.one_m_g;
+  const float b = expf(s * ph.k_exp);
+  const float db = b * ds * ph.k_exp;
   const float rho = a * b;
   const float drho = da * b + a * db;
-  const float dP = -(drho / e.w - rho * d.w / (e.w * e.w));
+  const float dP = -(drho * ie - rho * d.w * (ie * ie));
   const float kin = u * u + v * v;
   const float dkin = 2.0f * u * du + 2.0f * v * dv;
   out[0] = drho;
   out[1] = drho * u + rho * du;
   out[2] = drho * v + rho * dv;
-  out[3] = dP / ph.gm1 + 0.5f * (drho * kin + rho * dkin);
+  out[3] = dP * ph.inv_gm1 + 0.5f * (drho * kin + rho * dkin);
 }
 
 __device__ __forceinline__ float log_mean(float aL, float aR) {
-  const float xi = aL / aR;
-  const float f = (xi - 1.0f) / (xi + 1.0f);
+  const float xi = fdiv(aL, aR);
+  const float f = fdiv(xi - 1.0f, xi + 1.0f);
   const float u = f * f;
-  const float F = (u < 1e-3f) ? 1.0f + u / 3.0f + (u * u) / 5.0f + (u * (u * u)) / 7.0f : logf(xi) / (2.0f * f);
-  return (aL + aR) / (2.0f * F);
+  const float F = (u < 1e-3f) ? 1.0f + u * (1.0f / 3.0f) + (u * u) * 0.2f + (u * (u * u)) * (1.0f / 7.0f)
+                              : fdiv(logf(xi), 2.0f * f);
+  return fdiv(aL + aR, 2.0f * F);
 }
 
 // Ismail-Roe entropy-conservative flux (euler_solver.py:532-588)
@@ -377,19 +358,19 @@ __device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR
                                                 float F[4]) {
   float ZL[4], ZR[4];
   {
-    const float r = WL.x, u = WL.y / r, v = WL.z / r;
+    const float r = WL.x, ir = frcp(r), u = WL.y * ir, v = WL.z * ir;
     const float P = (WL.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
-    const float s = sqrtf(r / P);
-    ZL[0] = s * 1.0f;
+    const float s = __builtin_sqrtf(fdiv(r, P));
+    ZL[0] = s;
     ZL[1] = s * u;
     ZL[2] = s * v;
     ZL[3] = s * P;
   }
   {
-    const float r = WR.x, u = WR.y / r, v = WR.z / r;
+    const float r = WR.x, ir = frcp(r), u = WR.y * ir, v = WR.z * ir;
     const float P = (WR.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
-    const float s = sqrtf(r / P);
-    ZR[0] = s * 1.0f;
+    const float s = __builtin_sqrtf(fdiv(r, P));
+    ZR[0] = s;
     ZR[1] = s * u;
     ZR[2] = s * v;
     ZR[3] = s * P;
@@ -399,13 +380,14 @@ __device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR
   for (int v = 0; v < 4; ++v) Zb[v] = 0.5f * (ZL[v] + ZR[v]);
   const float Z1 = log_mean(ZL[0], ZR[0]);
   const float Z4 = log_mean(ZL[3], ZR[3]);
+  const float iZ0 = frcp(Zb[0]);
   const float rh = Zb[0] * Z4;
-  const float uh = Zb[1] / Zb[0];
-  const float vh = Zb[2] / Zb[0];
-  const float P1 = Zb[3] / Zb[0];
-  const float P2 = ph.c_ir1 * Z4 / Z1 + ph.c_ir2 * Zb[3] / Zb[0];
-  const float ah = sqrtf(ph.g * P2 / rh);
-  const float Hh = ah * ah / ph.gm1 + (uh * uh + vh * vh) / 2.0f;
+  const float uh = Zb[1] * iZ0;
+  const float vh = Zb[2] * iZ0;
+  const float P1 = Zb[3] * iZ0;
+  const float P2 = fdiv(ph.c_ir1 * Z4, Z1) + ph.c_ir2 * Zb[3] * iZ0;
+  const float ah2 = fdiv(ph.g * P2, rh);
+  const float Hh = ah2 * ph.inv_gm1 + (uh * uh + vh * vh) * 0.5f;
   const float un = uh * nx + vh * ny;
   F[0] = rh * un;
   F[1] = rh * uh * un + P1 * nx;
@@ -418,22 +400,23 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float
   // Roe average with clamps (helper.py:101-127)
   float rL = WL.x;
   rL = (rL <= 0.0f) ? 1e-6f : rL;
-  const float uL = WL.y / rL, vL = WL.z / rL;
+  const float irL = frcp(rL), uL = WL.y * irL, vL = WL.z * irL;
   float PL = (WL.w - 0.5f * rL * (uL * uL + vL * vL)) * ph.gm1;
   PL = (PL <= 0.0f) ? 1e-6f : PL;
   float rR = WR.x;
   rR = (rR <= 0.0f) ? 1e-6f : rR;
-  const float uR = WR.y / rR, vR = WR.z / rR;
+  const float irR = frcp(rR), uR = WR.y * irR, vR = WR.z * irR;
   float PR = (WR.w - 0.5f * rR * (uR * uR + vR * vR)) * ph.gm1;
   PR = (PR <= 0.0f) ? 1e-6f : PR;
-  const float sL = sqrtf(rL), sR = sqrtf(rR);
+  const float sL = __builtin_sqrtf(rL), sR = __builtin_sqrtf(rR);
+  const float isum = frcp(sL + sR);
   const float r = sL * sR;
-  const float u = (sL * uL + sR * uR) / (sL + sR);
-  const float v = (sL * vL + sR * vR) / (sL + sR);
-  const float HL = (WL.w + PL) / rL;
-  const float HR = (WR.w + PR) / rR;
-  const float H = (sL * HL + sR * HR) / (sL + sR);
-  const float a = sqrtf(fabsf(ph.gm1 * (H - 0.5f * (u * u + v * v))));
+  const float u = (sL * uL + sR * uR) * isum;
+  const float v = (sL * vL + sR * vR) * isum;
+  const float HL = (WL.w + PL) * irL;
+  const float HR = (WR.w + PR) * irR;
+  const float H = (sL * HL + sR * HR) * isum;
+  const float a = __builtin_sqrtf(fabsf(ph.gm1 * (H - 0.5f * (u * u + v * v))));
   // eigenvectors R (columns), euler_solver.py:156-175
   float Rm[4][4];
   Rm[0][0] = 1.0f;
@@ -454,7 +437,8 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float
   Rm[3][3] = H + a * (u * nx + v * ny);
   const float un = u * nx + v * ny;
   const float lam[4] = {un - a, un, un, un + a};
-  const float S[4] = {r / ph.two_g, ph.gm1 * r / ph.g, r / ph.g, r / ph.two_g};
+  const float ig = frcp(ph.g), i2g = frcp(ph.two_g);
+  const float S[4] = {r * i2g, ph.gm1 * r * ig, r * ig, r * i2g};
   const float4 eR = entropy_vars(WR, ph);
   const float4 eL = entropy_vars(WL, ph);
   const float dv[4] = {eR.x - eL.x, eR.y - eL.y, eR.z - eL.z, eR.w - eL.w};
@@ -485,19 +469,19 @@ __device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const 
                                           float ny, const Phys& ph, float D[4]) {
   const float unL = L.u * nx + L.v * ny;
   const float unR = R.u * nx + R.v * ny;
-  const float aL = sqrtf(fabsf(ph.g * L.P / L.r)) * ph.invM;
-  const float aR = sqrtf(fabsf(ph.g * R.P / R.r)) * ph.invM;
+  const float aL = sound(L, ph);
+  const float aR = sound(R, ph);
   const float af = 0.5f * (aL + aR);
-  const float MnL = unL / (aL + 1e-12f);
-  const float MnR = unR / (aR + 1e-12f);
+  const float MnL = fdiv(unL, aL + 1e-12f);
+  const float MnR = fdiv(unR, aR + 1e-12f);
   float MpL, MmL_, PpL, PmL_, MpR_, MmR, PpR_, PmR;
   ausm_split(MnL, MpL, MmL_, PpL, PmL_);
   ausm_split(MnR, MpR_, MmR, PpR_, PmR);
   const float mL = af * MpL * L.r;
   const float mR = af * MmR * R.r;
   const float pf = PpL * L.P + PmR * R.P;
-  const float HL = (WL.w + L.P * ph.invM2) / L.r;
-  const float HR = (WR.w + R.P * ph.invM2) / R.r;
+  const float HL = fdiv(WL.w + L.P * ph.invM2, L.r);
+  const float HR = fdiv(WR.w + R.P * ph.invM2, R.r);
   float FA[4];
   FA[0] = mL + mR;
   FA[1] = mL * L.u + mR * R.u + pf * nx * ph.invM2;
@@ -511,7 +495,7 @@ __device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const 
 }
 
 template <int FLUX, int ENT>
-__device__ __forceinline__ void side_flux(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
+__device__ __forceinline__ void face_flux(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
                                           float lf_c, float F[4]) {
   const Prim L = prim_of(WL, ph.M2, ph.gm1);
   const Prim R = prim_of(WR, ph.M2, ph.gm1);
@@ -558,76 +542,116 @@ __device__ __forceinline__ void side_flux(const float4 WL, const float4 WR, floa
   for (int v = 0; v < 4; ++v) F[v] = Fc[v] - D[v];
 }
 
-// Left and right conserved states of local face k of a work-item.
+// Conserved left (owner) and right (partner or ghost) states of a logical face.
 template <int RECON>
-__device__ __forceinline__ void face_states(const Item& it, int k, uint32_t f, const float4 Wi, const CaseDev& cs,
+__device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int32_t c, const CaseDev& cs,
                                             const float4* __restrict__ Ws, const float4* __restrict__ rec,
                                             float4& WL, float4& WR) {
-  const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
-  const int32_t n = m.n;
+  const uint32_t own = m.f_own[face];
+  const uint32_t f = m.f_ref[face];
+  const int32_t i = (int32_t)(own & FREF_IDX);
+  const int k = (int)(own >> 30);
   if (RECON == ESR_RECON_MUSCL) {
-    WL = cons_of(as_prim(rec[ridx_of(G, k, it.cell, it.c)]), ph.gm1M2);
+    WL = cons_of(as_prim(rec[ridx_of(G, k, i, c)]), ph.inv_gm1M2);
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + it.cell] : make_float2(0.f, 0.f);
+      const float4 Wi = Ws[sidx_of(G, i, c)];
+      const float2 nk = (kind == G_WALL) ? m.f_nrm[face] : make_float2(0.f, 0.f);
       const float4 Wg = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws,
-                                    sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
-      WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.gm1M2);
+                                    sidx_of(G, (int32_t)(f & FREF_BIDX), c));
+      WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.inv_gm1M2);
     } else {
-      const int32_t j = (int32_t)(f & FREF_IDX);
-      const int kk = (int)(f >> 30);
-      WR = cons_of(as_prim(rec[ridx_of(G, kk, j, it.c)]), ph.gm1M2);
+      WR = cons_of(as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)]), ph.inv_gm1M2);
     }
   } else {
-    WL = Wi;
+    WL = Ws[sidx_of(G, i, c)];
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + it.cell] : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+      const float2 nk = (kind == G_WALL) ? m.f_nrm[face] : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, WL, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), c));
     } else {
-      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), c)];
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// flux + residual + stage update
+// per-face flux: F * len of each logical face, owner's orientation
 // ---------------------------------------------------------------------------
 
-template <int RECON, int FLUX, int ENT, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_flux(const GroupDev* __restrict__ groups,
+template <int RECON, int FLUX, int ENT>
+__global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ groups,
                                                 const int32_t* __restrict__ blk_group,
-                                                const CaseDev* __restrict__ cases,
-                                                const float4* __restrict__ Ws, const float4* W0,
-                                                const float4* __restrict__ rec, float4* out, float4* acc) {
+                                                const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
+                                                const float4* __restrict__ rec, float4* __restrict__ fx) {
   Item it;
-  if (!map_item(groups, blk_group, cases, it)) return;
+  if (!map_item<true>(groups, blk_group, cases, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
-  const int32_t n = m.n, cell = it.cell;
   const CaseDev& cs = cases[G->case0 + it.c];
+  float4 WL, WR;
+  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, WR);
+  const float2 nk = m.f_nrm[it.idx];
+  const float sk = m.f_len[it.idx];
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
+  float F[4];
+  face_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, G->ph, lf_c, F);
+  fx[fidx_of(G, it.idx, it.c)] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
+}
 
-  const float4 Wi = Ws[it.sidx];
-  float Fs[3][4];
+// Lax-Friedrichs: per-case maximum wave speed over both states of every face
+// (euler_solver.py:267-269), integer max on the float bits (values >= 0).
+template <int RECON>
+__global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ groups,
+                                                   const int32_t* __restrict__ blk_group, CaseDev* cases,
+                                                   const float4* __restrict__ Ws, const float4* __restrict__ rec) {
+  Item it;
+  if (!map_item<true>(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const Phys& ph = G->ph;
+  CaseDev& cs = cases[G->case0 + it.c];
+  float4 WL, WR;
+  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, WR);
+  const float2 nk = G->m.f_nrm[it.idx];
+  const Prim L = prim_of(WL, ph.M2, ph.gm1), R = prim_of(WR, ph.M2, ph.gm1);
+  const float cl = wave_speed(L, nk.x, nk.y, ph), cr = wave_speed(R, nk.x, nk.y, ph);
+  const float cm = (cl != cl) ? cl : ((cr != cr) ? cr : fmaxf(cl, cr));
+  atomicMax(&cs.lf_cmax, __float_as_uint(cm));
+}
+
+// ---------------------------------------------------------------------------
+// deterministic face-to-cell gather + residual + stage update
+// ---------------------------------------------------------------------------
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ groups,
+                                                  const int32_t* __restrict__ blk_group,
+                                                  const CaseDev* __restrict__ cases,
+                                                  const float4* __restrict__ Ws, const float4* W0,
+                                                  const float4* __restrict__ fx, float4* out, float4* acc) {
+  Item it;
+  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const int32_t n = m.n, cell = it.idx;
+  const CaseDev& cs = cases[G->case0 + it.c];
+
+  float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[(int64_t)k * n + cell];
-    float4 WL, WR;
-    face_states<RECON>(it, k, f, Wi, cs, Ws, rec, WL, WR);
-    const float2 nk = m.nrm[(int64_t)k * n + cell];
-    const float sk = m.len[(int64_t)k * n + cell];
-    float F[4];
-    side_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, G->ph, lf_c, F);
-#pragma unroll
-    for (int v = 0; v < 4; ++v) Fs[k][v] = sk * F[v];
+    const uint32_t cf = m.cface[(int64_t)k * n + cell];
+    const float4 f = fx[fidx_of(G, (int32_t)(cf & ~CFACE_NEG), it.c)];
+    Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
-  const float ar = m.area[cell];
+  const float4 Wi = Ws[it.sidx];
+  const float ia = m.inv_area[cell];
   float R[4];
-#pragma unroll
-  for (int v = 0; v < 4; ++v) R[v] = ((Fs[0][v] + Fs[1][v]) + Fs[2][v]) / ar;
+  R[0] = ((Fk[0].x + Fk[1].x) + Fk[2].x) * ia;
+  R[1] = ((Fk[0].y + Fk[1].y) + Fk[2].y) * ia;
+  R[2] = ((Fk[0].z + Fk[1].z) + Fk[2].z) * ia;
+  R[3] = ((Fk[0].w + Fk[1].w) + Fk[2].w) * ia;
   if (G->sponge) {
     const float sigma = cs.strength * G->sponge_g[cell];
     R[0] = R[0] - sigma * (cs.value.x - Wi.x);
@@ -670,34 +694,6 @@ __global__ __launch_bounds__(BLOCK) void k_flux(const GroupDev* __restrict__ gro
                     w0.w - h * (a.w + R[3]));
   }
   out[s] = o;
-}
-
-// Lax-Friedrichs: per-case maximum wave speed over every side-face state
-// (euler_solver.py:267-269), integer max on the float bits (values >= 0).
-template <int RECON>
-__global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ groups,
-                                                   const int32_t* __restrict__ blk_group, CaseDev* cases,
-                                                   const float4* __restrict__ Ws, const float4* __restrict__ rec) {
-  Item it;
-  if (!map_item(groups, blk_group, cases, it)) return;
-  const GroupDev* G = it.G;
-  const MeshDev& m = G->m;
-  const Phys& ph = G->ph;
-  CaseDev& cs = cases[G->case0 + it.c];
-  const float4 Wi = Ws[it.sidx];
-  float cm = 0.0f;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[(int64_t)k * m.n + it.cell];
-    float4 WL, WR;
-    face_states<RECON>(it, k, f, Wi, cs, Ws, rec, WL, WR);
-    const float2 nk = m.nrm[(int64_t)k * m.n + it.cell];
-    const Prim L = prim_of(WL, ph.M2, ph.gm1), R = prim_of(WR, ph.M2, ph.gm1);
-    const float cl = wave_speed(L, nk.x, nk.y, ph), cr = wave_speed(R, nk.x, nk.y, ph);
-    const float c2 = (cl != cl) ? cl : ((cr != cr) ? cr : fmaxf(cl, cr));
-    cm = (c2 != c2) ? c2 : ((cm != cm) ? cm : fmaxf(cm, c2));
-  }
-  atomicMax(&cs.lf_cmax, __float_as_uint(cm));
 }
 
 // ---------------------------------------------------------------------------
@@ -854,8 +850,12 @@ __global__ __launch_bounds__(BLOCK) void k_forces(const GroupDev* __restrict__ g
 // Stationarity check of main.py:149-177 for every case whose check flag is set.
 __global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* __restrict__ part_d,
                         const double* __restrict__ part_r, const int32_t* __restrict__ nranges, int32_t rstride,
-                        const double* __restrict__ forces, int64_t step, int32_t K) {
+                        const double* __restrict__ forces, int64_t step, int32_t K, int64_t* dstep) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dstep) {
+    step = dstep[1];
+    if (c == 0) dstep[0] = step;
+  }
   if (c >= n_cases) return;
   CaseDev& cs = cases[c];
   CaseLoop& L = loops[c];
@@ -967,11 +967,19 @@ __global__ void k_count_active(int32_t n_cases, const CaseDev* __restrict__ case
 }
 
 __global__ void k_set_case(CaseDev* dst, CaseDev v) { *dst = v; }
+__global__ void k_set_step(int64_t* dstep, int64_t v) {
+  dstep[0] = v;
+  dstep[1] = v;
+}
 __global__ void k_set_group(GroupDev* dst, GroupDev v) { *dst = v; }
 
 __global__ void k_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* __restrict__ loops, int64_t step,
-                             int32_t K, int32_t all) {
+                             int32_t K, int32_t all, int64_t* dstep) {
   const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (dstep) {
+    step = dstep[0] + K;
+    if (c == 0) dstep[1] = step;
+  }
   if (c >= n_cases) return;
   if (all) {
     cases[c].check = 1;
@@ -999,7 +1007,7 @@ __global__ __launch_bounds__(BLOCK) void k_commit(const GroupDev* __restrict__ g
                                                   const CaseDev* __restrict__ cases, const float4* __restrict__ src,
                                                   float4* __restrict__ dst) {
   Item it;
-  if (!map_item(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, it)) return;
   dst[it.sidx] = src[it.sidx];
 }
 
@@ -1016,74 +1024,89 @@ void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, f
 }
 
 void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, hipStream_t st) {
-  if (ls.nblocks <= 0) return;
+  if (ls.fnblocks <= 0) return;
   CaseDev* cm = const_cast<CaseDev*>(cases);
   if (ls.recon == ESR_RECON_MUSCL)
-    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cm,
-                       Ws, rec);
-  else
-    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group,
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.fnblocks), dim3(BLOCK), 0, st, ls.groups, ls.fblk_group,
                        cm, Ws, rec);
+  else
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.fnblocks), dim3(BLOCK), 0, st, ls.groups,
+                       ls.fblk_group, cm, Ws, rec);
   check_launch("k_lf_cmax");
 }
 
-template <int RECON, int FLUX, int ENT>
-static void launch_flux_mode(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                             const float4* rec, float4* out, float4* acc, hipStream_t st) {
-#define ESR_FLUX_CASE(MD)                                                                                    \
-  case MD:                                                                                                   \
-    hipLaunchKernelGGL((k_flux<RECON, FLUX, ENT, MD>), dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, \
-                       ls.blk_group, cases, Ws, W0, rec, out, acc);                                          \
-    break;
-  switch (mode) {
-    ESR_FLUX_CASE(MODE_RESID)
-    ESR_FLUX_CASE(MODE_EE)
-    ESR_FLUX_CASE(MODE_S1)
-    ESR_FLUX_CASE(MODE_SRK2_2)
-    ESR_FLUX_CASE(MODE_RK2_2)
-    ESR_FLUX_CASE(MODE_RK4_1)
-    ESR_FLUX_CASE(MODE_RK4_2)
-    ESR_FLUX_CASE(MODE_RK4_3)
-    ESR_FLUX_CASE(MODE_RK4_4)
-    default:
-      throw Error(ESR_ERR_VALUE, "unknown stage mode");
-  }
-#undef ESR_FLUX_CASE
-}
-
 template <int RECON, int ENT>
-static void launch_flux_f(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                          const float4* rec, float4* out, float4* acc, hipStream_t st) {
+static void launch_face_f(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, float4* fx,
+                          hipStream_t st) {
+  const dim3 g(ls.fnblocks), b(BLOCK);
   switch (ls.flux) {
-    case ESR_FLUX_LF: launch_flux_mode<RECON, ESR_FLUX_LF, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    case ESR_FLUX_LF:
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_LF, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec, fx);
+      break;
     case ESR_FLUX_RUSANOV:
-      launch_flux_mode<RECON, ESR_FLUX_RUSANOV, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_RUSANOV, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws,
+                         rec, fx);
       break;
     case ESR_FLUX_TADMOR:
-      launch_flux_mode<RECON, ESR_FLUX_TADMOR, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_TADMOR, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws,
+                         rec, fx);
       break;
-    case ESR_FLUX_AUSM: launch_flux_mode<RECON, ESR_FLUX_AUSM, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
-    case ESR_FLUX_HLLC: launch_flux_mode<RECON, ESR_FLUX_HLLC, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
-    case ESR_FLUX_ROE: launch_flux_mode<RECON, ESR_FLUX_ROE, ENT>(ls, mode, cases, Ws, W0, rec, out, acc, st); break;
+    case ESR_FLUX_AUSM:
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_AUSM, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
+                         fx);
+      break;
+    case ESR_FLUX_HLLC:
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_HLLC, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
+                         fx);
+      break;
+    case ESR_FLUX_ROE:
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_ROE, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
+                         fx);
+      break;
     default: throw Error(ESR_ERR_VALUE, "unknown flux");
   }
 }
 
-void launch_flux(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                 const float4* rec, float4* out, float4* acc, hipStream_t st) {
-  if (ls.nblocks <= 0) return;
+void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, float4* fx,
+                 hipStream_t st) {
+  if (ls.fnblocks <= 0) return;
   if (ls.recon == ESR_RECON_MUSCL) {
     if (ls.entropy)
-      launch_flux_f<ESR_RECON_MUSCL, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      launch_face_f<ESR_RECON_MUSCL, 1>(ls, cases, Ws, rec, fx, st);
     else
-      launch_flux_f<ESR_RECON_MUSCL, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      launch_face_f<ESR_RECON_MUSCL, 0>(ls, cases, Ws, rec, fx, st);
   } else {
     if (ls.entropy)
-      launch_flux_f<ESR_RECON_CONSTANT, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      launch_face_f<ESR_RECON_CONSTANT, 1>(ls, cases, Ws, rec, fx, st);
     else
-      launch_flux_f<ESR_RECON_CONSTANT, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+      launch_face_f<ESR_RECON_CONSTANT, 0>(ls, cases, Ws, rec, fx, st);
   }
-  check_launch("k_flux");
+  check_launch("k_face");
+}
+
+void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                   const float4* fx, float4* out, float4* acc, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+#define ESR_UPD_CASE(MD)                                                                                   \
+  case MD:                                                                                                 \
+    hipLaunchKernelGGL((k_update<MD>), dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, \
+                       Ws, W0, fx, out, acc);                                                              \
+    break;
+  switch (mode) {
+    ESR_UPD_CASE(MODE_RESID)
+    ESR_UPD_CASE(MODE_EE)
+    ESR_UPD_CASE(MODE_S1)
+    ESR_UPD_CASE(MODE_SRK2_2)
+    ESR_UPD_CASE(MODE_RK2_2)
+    ESR_UPD_CASE(MODE_RK4_1)
+    ESR_UPD_CASE(MODE_RK4_2)
+    ESR_UPD_CASE(MODE_RK4_3)
+    ESR_UPD_CASE(MODE_RK4_4)
+    default:
+      throw Error(ESR_ERR_VALUE, "unknown stage mode");
+  }
+#undef ESR_UPD_CASE
+  check_launch("k_update");
 }
 
 static dim3 red_grid(const GroupDev& g) {
@@ -1111,9 +1134,9 @@ void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases
 
 void launch_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* part_d, const double* part_r,
                   const int32_t* nranges, int32_t rstride, const double* forces, int64_t step, int32_t K,
-                  hipStream_t st) {
+                  hipStream_t st, int64_t* dstep) {
   hipLaunchKernelGGL(k_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, part_d, part_r,
-                     nranges, rstride, forces, step, K);
+                     nranges, rstride, forces, step, K, dstep);
   check_launch("k_check");
 }
 
@@ -1141,14 +1164,20 @@ void launch_set_case(CaseDev* dst, const CaseDev& v, hipStream_t st) {
   check_launch("k_set_case");
 }
 
+void launch_set_step(int64_t* dstep, int64_t v, hipStream_t st) {
+  hipLaunchKernelGGL(k_set_step, dim3(1), dim3(1), 0, st, dstep, v);
+  check_launch("k_set_step");
+}
+
 void launch_set_group(GroupDev* dst, const GroupDev& v, hipStream_t st) {
   hipLaunchKernelGGL(k_set_group, dim3(1), dim3(1), 0, st, dst, v);
   check_launch("k_set_group");
 }
 
 void launch_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int64_t step, int32_t K, int32_t all,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_mark_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, step, K, all);
+                       hipStream_t st, int64_t* dstep) {
+  hipLaunchKernelGGL(k_mark_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, step, K, all,
+                     dstep);
   check_launch("k_mark_check");
 }
 

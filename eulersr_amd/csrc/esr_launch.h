@@ -9,8 +9,10 @@ namespace esr {
 // A launch covers a set of groups with the same kernel configuration.
 struct LaunchSet {
   const GroupDev* groups = nullptr;   // device array
-  const int32_t* blk_group = nullptr; // device array: block -> group index
+  const int32_t* blk_group = nullptr; // device array: cell block -> group index
   int32_t nblocks = 0;
+  const int32_t* fblk_group = nullptr; // device array: face block -> group index
+  int32_t fnblocks = 0;
   int32_t flux = ESR_FLUX_HLLC;
   int32_t recon = ESR_RECON_MUSCL;
   int32_t entropy = 0;
@@ -21,8 +23,10 @@ constexpr int BLOCK = 256;
 void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st);
 void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec,
                     hipStream_t st);
-void launch_flux(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                 const float4* rec, float4* out, float4* acc, hipStream_t st);
+void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, float4* fx,
+                 hipStream_t st);
+void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                   const float4* fx, float4* out, float4* acc, hipStream_t st);
 
 // Per-case reductions over one group (host-side group descriptor `g`,
 // device copy at `gdev`). Partial results at part[case * rstride + r].
@@ -37,7 +41,7 @@ void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases
 // Finalisation of a stationarity check for every case (main.py:149-177).
 void launch_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* part_d,
                   const double* part_r, const int32_t* nranges, int32_t rstride, const double* forces,
-                  int64_t step, int32_t K, hipStream_t st);
+                  int64_t step, int32_t K, hipStream_t st, int64_t* dstep = nullptr);
 // dt finalisation: dt = min * cfl; writes dt, dt/2, dt/6 into the case params.
 void launch_dt_final(int32_t n_cases, CaseDev* cases, const float* part, const int32_t* nranges,
                      int32_t rstride, const float* cfl, float* dt_out, hipStream_t st);
@@ -53,8 +57,11 @@ void launch_set_case(CaseDev* dst, const CaseDev& v, hipStream_t st);
 void launch_set_group(GroupDev* dst, const GroupDev& v, hipStream_t st);
 // check flags: all == 1 marks every case, otherwise active cases with
 // step % K == 0 or step == N.
+// dstep != nullptr: the step is dstep[0] + K (graph replays) and is
+// recorded in dstep[1]; k_check then reads dstep[1] and stores it in dstep[0].
 void launch_mark_check(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int64_t step, int32_t K, int32_t all,
-                       hipStream_t st);
+                       hipStream_t st, int64_t* dstep = nullptr);
+void launch_set_step(int64_t* dstep, int64_t v, hipStream_t st);
 // final one-step probe residual (main.py:228-230)
 void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, const double* part_r,
                         const int32_t* nranges, int32_t rstride, double* out, hipStream_t st);

@@ -188,6 +188,74 @@ esr_mesh* build_mesh(const esr_mesh_desc* d) {
       }
     }
 
+    // least-squares geometry (helper.py:136-153): offsets dl_k (neighbour
+    // barycentre, or twice the face-midpoint offset on marker > 0 faces),
+    // weights 1/|dl|^2 and the inverse of sum_k w dl dl^T, in double.
+    std::vector<float2> wdl(3 * (size_t)n);
+    std::vector<float4> ainv(n);
+    std::vector<float> iarea(n);
+    for (int32_t i = 0; i < n; ++i) {
+      double A00 = 0, A01 = 0, A11 = 0;
+      for (int k = 0; k < 3; ++k) {
+        const size_t s = (size_t)k * n + i;
+        float2 dl;
+        if (fref_is_bnd(fref[s]) || (fref[s] & FREF_PER)) {
+          dl = make_float2(2.0f * dxm[s].x, 2.0f * dxm[s].y);
+        } else {
+          const float2 bj = bary[fref[s] & FREF_IDX];
+          dl = make_float2(bj.x - bary[i].x, bj.y - bary[i].y);
+        }
+        const double x = dl.x, y = dl.y, w = 1.0 / (x * x + y * y);
+        wdl[s] = make_float2((float)(w * x), (float)(w * y));
+        A00 += w * x * x;
+        A01 += w * x * y;
+        A11 += w * y * y;
+      }
+      const double det = A00 * A11 - A01 * A01;
+      if (!(std::fabs(det) > 0.0)) throw Error(ESR_ERR_VALUE, "degenerate least-squares stencil");
+      ainv[i] = make_float4((float)(A11 / det), (float)(-A01 / det), (float)(-A01 / det), (float)(A00 / det));
+      iarea[i] = (float)(1.0 / (double)area[i]);
+    }
+
+    // logical faces. The owner of a shared face is the side whose ORIGINAL
+    // cell index is lower (then the lower slot), so the arithmetic of every
+    // face is independent of the renumbering; faces are stored in renumbered
+    // owner order for locality.
+    auto owns = [&](int32_t i, int k) -> bool {
+      const uint32_t w = fref[(size_t)k * n + i];
+      if (fref_is_bnd(w)) return true;
+      const int32_t j = (int32_t)(w & FREF_IDX);
+      const int kk = (int)(w >> 30);
+      const int32_t oi = m->perm[i], oj = m->perm[j];
+      return oi < oj || (oi == oj && k < kk);
+    };
+    std::vector<uint32_t> cface(3 * (size_t)n, 0xffffffffu), f_own, f_ref;
+    std::vector<float2> f_nrm;
+    std::vector<float> f_len;
+    for (int32_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) {
+        if (!owns(i, k)) continue;
+        const size_t s = (size_t)k * n + i;
+        const uint32_t id = (uint32_t)f_own.size();
+        f_own.push_back(((uint32_t)k << 30) | (uint32_t)i);
+        f_ref.push_back(fref[s]);
+        f_nrm.push_back(nrm[s]);
+        f_len.push_back(len[s]);
+        if (cface[s] != 0xffffffffu) throw Error(ESR_ERR_VALUE, "inconsistent face pairing");
+        cface[s] = id;
+        if (!fref_is_bnd(fref[s])) {
+          const uint32_t j = fref[s] & FREF_IDX, kk = fref[s] >> 30;
+          const size_t sp = (size_t)kk * n + j;
+          if (cface[sp] != 0xffffffffu || owns((int32_t)j, (int)kk))
+            throw Error(ESR_ERR_VALUE, "inconsistent face pairing");
+          cface[sp] = id | CFACE_NEG;
+        }
+      }
+    }
+    for (size_t s = 0; s < cface.size(); ++s)
+      if (cface[s] == 0xffffffffu) throw Error(ESR_ERR_VALUE, "a cell side has no logical face");
+    const int32_t nlf = (int32_t)f_own.size();
+
     // sponge extremes from all mesh points (helper.py:271-272)
     float ymin = P[1], ymax = P[1];
     for (int32_t p = 1; p < np; ++p) {
@@ -222,12 +290,19 @@ esr_mesh* build_mesh(const esr_mesh_desc* d) {
     }
 
     upload(m->b_fref, fref);
-    upload(m->b_bary, bary);
     upload(m->b_dxm, dxm);
+    upload(m->b_wdl, wdl);
+    upload(m->b_ainv, ainv);
     upload(m->b_nrm, nrm);
+    upload(m->b_fown, f_own);
+    upload(m->b_fref2, f_ref);
+    upload(m->b_fnrm, f_nrm);
+    upload(m->b_flen, f_len);
+    upload(m->b_cface, cface);
+    upload(m->b_iarea, iarea);
+    upload(m->b_sponge, g);
     upload(m->b_len, len);
     upload(m->b_area, area);
-    upload(m->b_sponge, g);
     upload(m->b_perm, m->perm);
     upload(m->b_wcell, wcell);
     upload(m->b_wnx, wnx);
@@ -236,23 +311,33 @@ esr_mesh* build_mesh(const esr_mesh_desc* d) {
 
     MeshDev& md = m->dev;
     md.n = n;
+    md.nf = nlf;
     md.n_wall = (int32_t)wcell.size();
     md.fref = m->b_fref.as<uint32_t>();
-    md.bary = m->b_bary.as<float2>();
     md.dxm = m->b_dxm.as<float2>();
+    md.wdl = m->b_wdl.as<float2>();
+    md.ainv = m->b_ainv.as<float4>();
     md.nrm = m->b_nrm.as<float2>();
+    md.f_own = m->b_fown.as<uint32_t>();
+    md.f_ref = m->b_fref2.as<uint32_t>();
+    md.f_nrm = m->b_fnrm.as<float2>();
+    md.f_len = m->b_flen.as<float>();
+    md.cface = m->b_cface.as<uint32_t>();
+    md.inv_area = m->b_iarea.as<float>();
+    md.sponge_g = m->b_sponge.as<float>();
     md.len = m->b_len.as<float>();
     md.area = m->b_area.as<float>();
-    md.sponge_g = m->b_sponge.as<float>();
     md.wall_cell = m->b_wcell.as<int32_t>();
     md.wall_nx = m->b_wnx.as<float>();
     md.wall_ny = m->b_wny.as<float>();
     md.wall_ds = m->b_wds.as<float>();
     const float* is = d->inlet_subsonic;
     md.inlet_subsonic = is ? make_float4(is[0], is[1], is[2], is[3]) : make_float4(1.f, 1.f, 0.f, 1.f);
-    m->device_bytes = (int64_t)(m->b_fref.bytes + m->b_bary.bytes + m->b_dxm.bytes + m->b_nrm.bytes +
-                                m->b_len.bytes + m->b_area.bytes + m->b_sponge.bytes + m->b_perm.bytes +
-                                m->b_wcell.bytes + m->b_wnx.bytes + m->b_wny.bytes + m->b_wds.bytes);
+    m->device_bytes = (int64_t)(m->b_fref.bytes + m->b_dxm.bytes + m->b_wdl.bytes + m->b_ainv.bytes +
+                                m->b_nrm.bytes + m->b_fown.bytes + m->b_fref2.bytes + m->b_fnrm.bytes +
+                                m->b_flen.bytes + m->b_cface.bytes + m->b_iarea.bytes + m->b_sponge.bytes +
+                                m->b_perm.bytes + m->b_wcell.bytes + m->b_wnx.bytes + m->b_wny.bytes +
+                                m->b_wds.bytes + m->b_len.bytes + m->b_area.bytes);
   } catch (...) {
     delete m;
     throw;

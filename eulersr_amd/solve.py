@@ -37,7 +37,8 @@ def transonic_cfl(mach: float, cfl: float) -> float:
 
 
 class BatchSolver:
-    def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0):
+    def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
+                 graph=True):
         require_gpu()
         self.specs = list(cases)
         n = len(self.specs)
@@ -65,6 +66,7 @@ class BatchSolver:
         so.patience = max(1, int(patience))
         so.max_steps = int(max_steps)
         so.local_dt = 0
+        so.no_graph = 0 if graph else 1
         h = C.c_void_p()
         L.check(L.lib().esr_solver_create(n, arr, C.byref(so), C.byref(h)), "esr_solver_create")
         self.handle = h
@@ -113,17 +115,17 @@ class BatchSolver:
                     diverged=bool(r.diverged))
 
     def info(self):
-        items, cells = C.c_int64(), C.c_int64()
-        L.check(L.lib().esr_solver_info(self.handle, C.byref(items), C.byref(cells)), "esr_solver_info")
-        return items.value, cells.value
+        """(cell work-items, group cells, face work-items, group faces) of one stage."""
+        v = [C.c_int64() for _ in range(4)]
+        L.check(L.lib().esr_solver_info(self.handle, *[C.byref(x) for x in v]), "esr_solver_info")
+        return tuple(x.value for x in v)
 
     def time_kernels(self, reps=20):
-        """Average ms per launch of (reconstruction, stage-1 flux) kernels,
-        HIP events on the current stream."""
-        a, b = C.c_double(), C.c_double()
-        L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), C.byref(a), C.byref(b), stream_ptr()),
-                "esr_solver_time_kernels")
-        return a.value, b.value
+        """Average ms per launch of the (reconstruction, face flux, gather +
+        update) kernels, HIP events on the current stream."""
+        ms = (C.c_double * 3)()
+        L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), ms, stream_ptr()), "esr_solver_time_kernels")
+        return tuple(ms)
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

@@ -152,6 +152,8 @@ typedef struct {
   int32_t patience;          /* stationarity_patience (>= 1)    */
   int64_t max_steps;         /* extra cap on steps (<= 0: none) */
   int32_t local_dt;          /* 0: global dt (reference). 1: local CFL dt (acceleration mode) */
+  int32_t no_graph;          /* 1: launch every kernel eagerly instead of replaying
+                                a captured HIP graph of check_every steps */
 } esr_solve_opts;
 
 typedef struct {
@@ -187,14 +189,16 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
 
 /* -- diagnostics ----------------------------------------------------------- */
 
-/* Work-items (cell x case pairs) and distinct group cells of one stage launch. */
-esr_status esr_solver_info(const esr_solver* s, int64_t* work_items, int64_t* group_cells);
-/* Average device time (ms) of one launch of the reconstruction kernel and of
- * the stage-1 flux/update kernel over the solver's current states, each
- * launched `reps` times back to back between two HIP events on `stream`.
- * Both launches are idempotent (they rewrite scratch and the stage buffer). */
-esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms_recon, double* ms_flux,
-                                   void* stream);
+/* Sizes of one stage launch: cell work-items (cell x case), distinct group
+ * cells, face work-items (logical face x case) and distinct group faces. */
+esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* group_cells, int64_t* face_items,
+                           int64_t* group_faces);
+/* Average device time (ms) of one launch of each stage kernel over the
+ * solver's current states: ms[0] reconstruction, ms[1] face flux, ms[2]
+ * gather + stage-1 update. Each is launched `reps` times back to back
+ * between two HIP events on `stream`; all three are idempotent (they
+ * rewrite scratch buffers and the stage buffer only). */
+esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
 const char* esr_last_error(void);
 int32_t esr_version(void);
 

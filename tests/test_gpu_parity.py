@@ -245,3 +245,43 @@ def test_batch_equals_single_cases_bitwise():
         single = one.get_state(0).cpu().numpy()
         print(f"case {i}: batched vs single", flush=True)
         assert np.array_equal(batched[i].view(np.uint32), single.view(np.uint32))
+
+
+def test_graph_replays_equal_eager_launches():
+    """The captured chunk (check_every steps + one stationarity check) replayed
+    many times, with eager work and host synchronisation between replays,
+    gives bit-identical states and identical stop bookkeeping to eager
+    launches; run() calls that end off a chunk boundary exercise the eager
+    fallback."""
+    from eulersr_amd import main as M
+    from eulersr_amd.solve import BatchSolver
+    m1 = mesh_of("diamond", 0.2)
+    specs, inits = [], []
+    for mach, aoa in [(2.0, 3.0), (1.5, -2.0), (3.0, 0.0)]:
+        cfg = base_cfg(Mach=mach, aoa=aoa, tf=0.5)
+        W, inlet = M.initialize(m1, cfg)
+        sp = M.case_spec(m1, inlet, cfg)
+        specs.append(sp)
+        inits.append(W)
+    specs[1].threshold = 1.5e-3   # stops early: exercises the per-case mask
+    solvers = []
+    for graph in (True, False):
+        bs = BatchSolver(specs, scheme="SRK2", threshold=1e-9, check_every=5, patience=2, graph=graph)
+        for i, W in enumerate(inits):
+            bs.set_state(i, W)
+        bs.prepare()
+        solvers.append(bs)
+    for until in (23, 60, 61, 140, 0):
+        outs = []
+        for bs in solvers:
+            bs.run(until)
+            outs.append(([bs.get_state(i).cpu().numpy() for i in range(3)], [bs.report(i) for i in range(3)]))
+        (sg, rg), (se, re_) = outs
+        print(f"case: chunk until {until}: steps {solvers[0].steps} {solvers[1].steps}", flush=True)
+        assert solvers[0].steps == solvers[1].steps
+        for a, b in zip(sg, se):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        for a, b in zip(rg, re_):
+            for key in ("converged", "stopping_step", "steps_run", "last_rel"):
+                assert a[key] == b[key], (key, a[key], b[key])
+    assert rg[1]["converged"] and not rg[0]["converged"]
