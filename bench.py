@@ -312,6 +312,16 @@ def main():
         mean_steps = float(np.mean([s for s, r in zip(steps_case, reps) if r["converged"]] or steps_case))
         out["cpu_baseline"] = cpu_baseline(meshes["diamond"], specs[2], mean_steps)
     last.close()
+    if args.profile:
+        # counter calibration: a plain device copy of known size (1 GiB,
+        # larger than every cache), copied twice
+        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        for _ in range(2):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes_each_way"] = x.numel() * 4
+        del x, y
     print(json.dumps(out), flush=True)
     return 0
 

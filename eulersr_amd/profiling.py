@@ -1,0 +1,85 @@
+"""Summaries of rocprofv3 CSV output for the solver kernels.
+
+    python -m eulersr_amd.profiling trace <dir>/<name>_kernel_trace.csv
+    python -m eulersr_amd.profiling pmc <fetch_counter_collection.csv> <write_counter_collection.csv>
+
+``trace``: per kernel, the average duration over all dispatches and over the
+last ``--tail`` dispatches (bench.py ends with back-to-back launches of each
+stage kernel, the same launches it times with HIP events).
+``pmc``: per kernel, FETCH_SIZE and WRITE_SIZE (KiB per dispatch), and the
+same counters for the state-copy kernel k_copy, whose algorithmic traffic is
+known (16 B read + 16 B written per element), to give a calibration factor.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import re
+import statistics
+from collections import defaultdict
+
+
+def short(name: str) -> str:
+    m = re.match(r"(?:void )?esr::(k_\w+)(<[^>]*>)?", name)
+    return (m.group(1) + (m.group(2) or "")) if m else name[:40]
+
+
+def trace_summary(path, tail=50):
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    by = defaultdict(list)
+    for r in rows:
+        by[short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = []
+    out.append(f"{'kernel':34s} {'calls':>7s} {'avg_us':>9s} {'tail_avg_us':>12s} {'grid':>10s}")
+    grids = {}
+    for r in rows:
+        grids[short(r["Kernel_Name"])] = r.get("Grid_Size_X", "")
+    for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+        t = v[-tail:] if len(v) >= tail else v
+        out.append(f"{k:34s} {len(v):7d} {statistics.mean(v):9.2f} {statistics.mean(t):12.2f} {grids[k]:>10s}")
+    return "\n".join(out)
+
+
+def pmc_summary(fetch_csv, write_csv, copy_bytes=None):
+    def load(path, counter):
+        vals = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r.get("Counter_Name", counter) != counter:
+                continue
+            vals[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+        return vals
+    f = load(fetch_csv, "FETCH_SIZE")
+    w = load(write_csv, "WRITE_SIZE")
+    out = [f"{'kernel':34s} {'calls':>7s} {'FETCH_KiB':>11s} {'WRITE_KiB':>11s}"]
+    for k in sorted(set(f) | set(w)):
+        fv = statistics.mean(f[k]) if f.get(k) else float("nan")
+        wv = statistics.mean(w[k]) if w.get(k) else float("nan")
+        out.append(f"{k:34s} {len(f.get(k, [])):7d} {fv:11.1f} {wv:11.1f}")
+    if copy_bytes and f.get("k_copy"):
+        fac_f = copy_bytes / 2 / 1024 / statistics.mean(f["k_copy"])
+        fac_w = copy_bytes / 2 / 1024 / statistics.mean(w["k_copy"])
+        out.append(f"k_copy calibration: algorithmic {copy_bytes / 2 / 1024:.1f} KiB each way -> "
+                   f"factor fetch {fac_f:.3f}, write {fac_w:.3f}")
+    return "\n".join(out)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("csv")
+    t.add_argument("--tail", type=int, default=50)
+    p = sub.add_parser("pmc")
+    p.add_argument("fetch_csv")
+    p.add_argument("write_csv")
+    p.add_argument("--copy-bytes", type=float, default=None)
+    a = ap.parse_args()
+    if a.cmd == "trace":
+        print(trace_summary(a.csv, a.tail))
+    else:
+        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes))
+
+
+if __name__ == "__main__":
+    main()
