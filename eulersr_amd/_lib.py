@@ -55,6 +55,13 @@ class SolveOpts(C.Structure):
     ]
 
 
+class KnnMap(C.Structure):
+    _fields_ = [
+        ("n_hr", C.c_int32), ("n_lr", C.c_int32), ("k", C.c_int32), ("knn_idx", C.c_void_p),
+        ("knn_rel", C.c_void_p), ("knn_w", C.c_void_p), ("wall", C.c_void_p),
+    ]
+
+
 class CaseReport(C.Structure):
     _fields_ = [
         ("dt", C.c_float), ("n_steps", C.c_int64), ("converged", C.c_int32),
@@ -92,6 +99,20 @@ SIGNATURES = {
     "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
+    "esr_mlp_create": (C.c_int, [C.c_int32, i32p, C.c_int64, C.POINTER(C.c_void_p)]),
+    "esr_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "esr_mlp_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "esr_mlp_set_params": (C.c_int, [C.c_void_p, f32p]),
+    "esr_mlp_get_params": (C.c_int, [C.c_void_p, f32p]),
+    "esr_mlp_forward": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "esr_mlp_loss_grad": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
+                                    f32p, C.c_void_p]),
+    "esr_mlp_adamw": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                C.c_void_p]),
+    "esr_warm_features": (C.c_int, [C.POINTER(KnnMap), C.c_int32, C.c_void_p, C.c_void_p, f32p, f32p,
+                                    C.c_double, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "esr_warm_apply": (C.c_int, [C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, f32p, C.c_double, C.c_void_p,
+                                 C.c_void_p, C.c_void_p]),
     "esr_last_error": (C.c_char_p, []),
     "esr_version": (C.c_int32, []),
 }

@@ -187,6 +187,54 @@ esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, voi
 esr_status esr_solver_finalize(esr_solver* s, void* stream);
 esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out);
 
+/* -- learned warm start ----------------------------------------------------- */
+/* The reference's warm start is a Flax network over kNN/IDW features
+ * (models/dam.py:302-304: IDW baseline + learned delta; kNN k=6 LR->HR,
+ * preprocessing/knn.py:20-40; IDW p=2, utils/metrics.py:9-14; wall features,
+ * utils/aero.py:154-181). Here: a multilayer perceptron (ReLU hidden layers,
+ * linear output) trained and run by hand-written BF16 MFMA kernels with FP32
+ * accumulation. Parameters are float32, per layer W (out x in, row-major)
+ * then b (out). X is bfloat16 (rows x widths[0]) on the device. */
+typedef struct esr_mlp esr_mlp;
+
+esr_status esr_mlp_create(int32_t n_layers, const int32_t* widths, int64_t max_rows, esr_mlp** out);
+esr_status esr_mlp_destroy(esr_mlp* m);
+esr_status esr_mlp_param_count(const esr_mlp* m, int64_t* n);
+esr_status esr_mlp_set_params(esr_mlp* m, const float* host);   /* also resets the AdamW moments */
+esr_status esr_mlp_get_params(const esr_mlp* m, float* host);
+/* Y (rows x widths[n_layers], float32, device) = network(X); any row count. */
+esr_status esr_mlp_forward(esr_mlp* m, int64_t rows, const uint16_t* X, float* Y, void* stream);
+/* Forward, mean-squared error against T (rows x out, float32, device) and
+ * backward; gradients kept on the device (and copied to grads_host if not
+ * null); rows <= max_rows. Synchronises (returns the loss). */
+esr_status esr_mlp_loss_grad(esr_mlp* m, int64_t rows, const uint16_t* X, const float* T, double* loss,
+                             float* grads_host, void* stream);
+/* One AdamW update (decoupled weight decay on W only) with step count t >= 1. */
+esr_status esr_mlp_adamw(esr_mlp* m, float lr, float beta1, float beta2, float eps, float weight_decay,
+                         int32_t t, void* stream);
+
+/* LR -> HR neighbour map, device arrays in original cell order. */
+typedef struct {
+  int32_t n_hr, n_lr, k;
+  const int32_t* knn_idx;   /* (n_hr, k) LR cell of each neighbour          */
+  const float* knn_rel;     /* (n_hr, k, 2) (LR barycentre - HR) / h_LR      */
+  const float* knn_w;       /* (n_hr, k) normalised IDW weights              */
+  const float* wall;        /* (n_hr, 4) [wd/L, exp(-wd/(2 h_LR)), n_x, n_y] */
+} esr_knn_map;
+
+/* Network inputs for n_cases LR states (n_cases x n_lr x 4 conserved,
+ * device): X (n_cases*n_hr rows x in_pad, bf16) and the IDW prediction
+ * (n_cases*n_hr x 4 primitives). case_scal: (n_cases, 2) normalised
+ * (Mach, alpha) on the device; mu/sig: host primitive normalisation. */
+esr_status esr_warm_features(const esr_knn_map* map, int32_t n_cases, const float* lr_states,
+                             const float* case_scal, const float* mu, const float* sig, double gamma,
+                             int32_t in_pad, uint16_t* X, float* idw, void* stream);
+/* Warm-start state: prim = idw + sig * delta (delta null: IDW only) ->
+ * conserved W_out (rows x 4); bad[case] |= 1 when a cell of the case has
+ * rho <= 0, p <= 0 or a non-finite value (the freestream fallback test). */
+esr_status esr_warm_apply(int64_t rows, int32_t n_hr, const float* idw, const float* delta, const float* sig,
+                          double gamma, float* W_out, int32_t* bad, void* stream);
+
 /* -- diagnostics ----------------------------------------------------------- */
 
 /* Sizes of one stage launch: cell work-items (cell x case), distinct group
