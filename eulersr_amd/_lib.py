@@ -68,7 +68,7 @@ class CaseReport(C.Structure):
         ("stopping_step", C.c_int64), ("steps_run", C.c_int64), ("last_rel", C.c_double),
         ("final_residual", C.c_double), ("cd", C.c_double), ("cl", C.c_double),
         ("engineering_converged", C.c_int32), ("engineering_stopping_step", C.c_int64),
-        ("diverged", C.c_int32),
+        ("diverged", C.c_int32), ("recent_rel_max", C.c_double),
     ]
 
 

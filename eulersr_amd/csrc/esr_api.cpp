@@ -853,11 +853,18 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
     out->steps_run = L.done ? L.steps_run : s->step;
     out->last_rel = L.last_rel;
     out->final_residual = s->final_residual[c];
-    out->cd = s->cd[c];
-    out->cl = s->cl[c];
+    // after finalize: forces of the final state; before: the last in-loop
+    // check's (cases tracking the engineering criterion), else NaN
+    const bool fin = !std::isnan(s->cd[c]);
+    out->cd = fin ? s->cd[c] : (L.track_eng ? L.cd : s->cd[c]);
+    out->cl = fin ? s->cl[c] : (L.track_eng ? L.cl : s->cl[c]);
     out->engineering_converged = L.eng_stop;
     out->engineering_stopping_step = L.eng_stopping_step;
     out->diverged = L.diverged;
+    const int32_t nr = std::min(std::min(L.patience, L.n_checks), 4);
+    double rm = nr > 0 ? 0.0 : NAN;
+    for (int32_t q = 0; q < nr; ++q) rm = std::max(rm, L.recent[q]);
+    out->recent_rel_max = rm;
   });
 }
 

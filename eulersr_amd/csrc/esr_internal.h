@@ -141,7 +141,8 @@ struct CaseLoop {
   int32_t eng_patience;
   int32_t eng_stop;
   int32_t diverged;
-  int32_t pad;
+  int32_t n_checks;
+  double recent[4];  // the last four check residuals, newest first
 };
 
 enum StageMode : int {

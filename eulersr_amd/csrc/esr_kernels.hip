@@ -877,6 +877,11 @@ __global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const 
   L.streak_start = new_start;
   L.ref_step = step;
   L.last_rel = rel;
+  L.recent[3] = L.recent[2];
+  L.recent[2] = L.recent[1];
+  L.recent[1] = L.recent[0];
+  L.recent[0] = rel;
+  L.n_checks += 1;
   if (!(d == d) || isinf(d)) L.diverged = 1;
   if (L.track_eng && forces) {
     const double cd = forces[2 * (int64_t)c] / L.q_ref, cl = forces[2 * (int64_t)c + 1] / L.q_ref;
@@ -892,7 +897,8 @@ __global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const 
     L.eng_streak_start = st;
   }
   L.steps_run = step;
-  if (L.stop || step >= L.N) {
+  // a non-finite residual ends the case (unconverged) instead of running to N
+  if (L.stop || L.diverged || step >= L.N) {
     L.done = 1;
     cs.active = 0;
   }

@@ -17,10 +17,17 @@ struct GemmArgs {
   uint16_t* C16;      // bf16 output (row stride ldc16) or null
   int64_t ldc16;
   int32_t relu;       // ReLU on the bf16 output
+  int64_t kchunk;     // split-K chunk (0: no split); partial z at C32 + z*c_split
+  int64_t c_split;
+  int32_t mode_a, mode_b;  // operand staging modes (set by gemm_bf16)
 };
 
+constexpr int32_t kMaxSplits = 64;
+
 void gemm_bf16(const GemmArgs& g, hipStream_t st);
-void colsum(const float* X, int64_t rows, int32_t n, int64_t ld, float* out, hipStream_t st);
+void sum_splits(const float* part, int32_t ns, int64_t n, float* out, hipStream_t st);
+int32_t colsum_splits(int64_t rows);
+void colsum(const float* X, int64_t rows, int32_t n, int64_t ld, float* part, float* out, hipStream_t st);
 int32_t mse_blocks(int64_t total);
 void mse(const float* Y, const float* T, int64_t total, float scale, float* dY32, uint16_t* dY16, double* part,
          hipStream_t st);

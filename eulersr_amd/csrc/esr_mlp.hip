@@ -31,6 +31,49 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 constexpr int TM = 64, TN = 64, TK = 32, TKP = TK + 8;
 
+// Stage a 64 x 32 tile T[r][k] of the operand X(r, k) = X[r*s_r + k*s_k]
+// for rows [r0, rmax) and k in [k0, kmax). mode 1: k contiguous (one 16-byte
+// load of 8 k per thread); mode 2: r contiguous (8 r per thread); mode 0:
+// element-wise. Out-of-range entries are zero.
+__device__ __forceinline__ void stage(uint16_t (*T)[TKP], const uint16_t* __restrict__ X, int64_t s_r, int64_t s_k,
+                                      int r0, int rmax, int64_t k0, int64_t kmax, int mode, int tid) {
+  if (mode == 1) {
+    const int r = tid >> 2, kk = (tid & 3) * 8;
+    const int gr = r0 + r;
+    const int64_t gk = k0 + kk;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (gr < rmax && gk + 8 <= kmax) {
+      v = *reinterpret_cast<const uint4*>(X + (int64_t)gr * s_r + gk);
+      *reinterpret_cast<uint4*>(&T[r][kk]) = v;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) T[r][kk + q] = (gr < rmax && gk + q < kmax) ? X[(int64_t)gr * s_r + gk + q] : (uint16_t)0;
+    }
+  } else if (mode == 2) {
+    const int k = tid >> 3, rr = (tid & 7) * 8;
+    const int64_t gk = k0 + k;
+    const int gr = r0 + rr;
+    if (gk < kmax && gr + 8 <= rmax) {
+      const uint4 v = *reinterpret_cast<const uint4*>(X + gk * s_k + gr);
+      const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) T[rr + q][k] = h[q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) T[rr + q][k] = (gk < kmax && gr + q < rmax) ? X[gk * s_k + gr + q] : (uint16_t)0;
+    }
+  } else {
+    for (int e = tid; e < TM * TK; e += 256) {
+      const int r = e / TK, k = e % TK;
+      const int gr = r0 + r;
+      const int64_t gk = k0 + k;
+      T[r][k] = (gr < rmax && gk < kmax) ? X[(int64_t)gr * s_r + gk * s_k] : (uint16_t)0;
+    }
+  }
+}
+
+// blockIdx.z selects a K chunk [z*kchunk, (z+1)*kchunk); its fp32 partial
+// goes to C32 + z*c_split (bias and bf16 output only when gridDim.z == 1).
 __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) uint16_t As[TM][TKP];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[TN][TKP];  // Bs[j][k]
@@ -38,23 +81,17 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int lr = lane & 15, lk = (lane >> 4) * 8;
+  const int64_t kbeg = (int64_t)blockIdx.z * g.kchunk;
+  const int64_t kend = kbeg + g.kchunk < g.K ? kbeg + g.kchunk : g.K;
   f32x4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < g.K; k0 += TK) {
-    for (int e = tid; e < TM * TK; e += 256) {
-      const int i = e / TK, k = e % TK;
-      const int gi = bm + i, gk = k0 + k;
-      As[i][k] = (gi < g.M && gk < g.K) ? g.A[(int64_t)gi * g.sa_i + (int64_t)gk * g.sa_k] : (uint16_t)0;
-    }
-    for (int e = tid; e < TN * TK; e += 256) {
-      const int j = e / TK, k = e % TK;
-      const int gj = bn + j, gk = k0 + k;
-      Bs[j][k] = (gj < g.N && gk < g.K) ? g.B[(int64_t)gk * g.sb_k + (int64_t)gj * g.sb_j] : (uint16_t)0;
-    }
+  for (int64_t k0 = kbeg; k0 < kend; k0 += TK) {
+    stage(As, g.A, g.sa_i, g.sa_k, bm, g.M, k0, kend, g.mode_a, tid);
+    stage(Bs, g.B, g.sb_j, g.sb_k, bn, g.N, k0, kend, g.mode_b, tid);
     __syncthreads();
     bf16x8 af[2], bfr[2];
 #pragma unroll
@@ -68,6 +105,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
       for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     __syncthreads();
   }
+  float* C32 = g.C32 ? g.C32 + (int64_t)blockIdx.z * g.c_split : nullptr;
   // D(row, col): row = 4*(lane/16) + r, col = lane % 16 within each 16x16 tile
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -80,19 +118,32 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
         if (i < g.M && j < g.N) {
           float v = acc[a][b][r];
           if (g.bias) v += g.bias[j];
-          if (g.C32) g.C32[(int64_t)i * g.ldc32 + j] = v;
+          if (C32) C32[(int64_t)i * g.ldc32 + j] = v;
           if (g.C16) g.C16[(int64_t)i * g.ldc16 + j] = f2bf(g.relu ? fmaxf(v, 0.f) : v);
         }
       }
 }
 
-// column sums of an fp32 (rows x n) matrix, one thread per column, fixed order
-__global__ void k_colsum(const float* __restrict__ X, int64_t rows, int32_t n, int64_t ld, float* __restrict__ out) {
+// out[e] = sum over s of part[s*n + e], s ascending
+__global__ void k_sum_splits(const float* __restrict__ part, int32_t ns, int64_t n, float* __restrict__ out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.f;
+  for (int32_t q = 0; q < ns; ++q) s += part[q * n + e];
+  out[e] = s;
+}
+
+// partial column sums of an fp32 (rows x n) matrix: block (x, y) sums rows
+// [y*chunk, (y+1)*chunk) of 64 columns into part[y*n + j], rows ascending
+__global__ void k_colsum_part(const float* __restrict__ X, int64_t rows, int32_t n, int64_t ld, int64_t chunk,
+                              float* __restrict__ part) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
+  const int64_t r0 = (int64_t)blockIdx.y * chunk;
+  const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
   float s = 0.f;
-  for (int64_t i = 0; i < rows; ++i) s += X[i * ld + j];
-  out[j] = s;
+  for (int64_t i = r0; i < r1; ++i) s += X[i * ld + j];
+  part[(int64_t)blockIdx.y * n + j] = s;
 }
 
 // MSE of (rows x n) predictions against targets; dY = 2 (Y - T) / (rows n)
@@ -219,16 +270,45 @@ __global__ void k_warm_apply(int64_t rows, int32_t n_hr, const float* __restrict
 // launch wrappers
 // ---------------------------------------------------------------------------
 
-void gemm_bf16(const GemmArgs& g, hipStream_t st) {
-  if (g.M <= 0 || g.N <= 0) return;
-  dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
+namespace {
+int stage_mode(const uint16_t* X, int64_t s_r, int64_t s_k) {
+  const bool al = (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
+  if (s_k == 1 && s_r % 8 == 0 && al) return 1;
+  if (s_r == 1 && s_k % 8 == 0 && al) return 2;
+  return 0;
+}
+}  // namespace
+
+void gemm_bf16(const GemmArgs& g0, hipStream_t st) {
+  if (g0.M <= 0 || g0.N <= 0) return;
+  GemmArgs g = g0;
+  if (g.kchunk <= 0 || g.kchunk >= g.K) g.kchunk = g.K > 0 ? g.K : 1;
+  const int32_t nz = (int32_t)((g.K + g.kchunk - 1) / g.kchunk);
+  if (nz > 1 && (g.bias || g.C16 || g.kchunk % TK != 0))
+    throw Error(ESR_ERR_VALUE, "split-K GEMM takes fp32 output only and a K chunk multiple of 32");
+  g.mode_a = stage_mode(g.A, g.sa_i, g.sa_k);
+  g.mode_b = stage_mode(g.B, g.sb_j, g.sb_k);
+  dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, nz > 0 ? nz : 1);
   hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, st, g);
   hip_check(hipGetLastError(), "k_gemm_bf16");
 }
 
-void colsum(const float* X, int64_t rows, int32_t n, int64_t ld, float* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_colsum, dim3((n + 63) / 64), dim3(64), 0, st, X, rows, n, ld, out);
-  hip_check(hipGetLastError(), "k_colsum");
+void sum_splits(const float* part, int32_t ns, int64_t n, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, ns, n, out);
+  hip_check(hipGetLastError(), "k_sum_splits");
+}
+
+int32_t colsum_splits(int64_t rows) {
+  const int64_t s = (rows + 1023) / 1024;
+  return (int32_t)(s < 1 ? 1 : (s > 256 ? 256 : s));
+}
+
+void colsum(const float* X, int64_t rows, int32_t n, int64_t ld, float* part, float* out, hipStream_t st) {
+  const int32_t ns = colsum_splits(rows);
+  const int64_t chunk = (rows + ns - 1) / ns;
+  hipLaunchKernelGGL(k_colsum_part, dim3((n + 63) / 64, ns), dim3(64), 0, st, X, rows, n, ld, chunk, part);
+  hip_check(hipGetLastError(), "k_colsum_part");
+  sum_splits(part, ns, n, out, st);
 }
 
 int32_t mse_blocks(int64_t total) {

@@ -6,6 +6,7 @@
 // Parameter layout (host and device): for each layer l, W_l (out x in,
 // row-major) then b_l (out).
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -24,6 +25,8 @@ struct esr_mlp {
   DevBuf P, G, M1, M2, P16;
   std::vector<std::unique_ptr<DevBuf>> Z, H;  // per layer pre-activation (fp32), hidden activation (bf16)
   DevBuf dA32, dA16, dB32, part;
+  DevBuf split;  // split-K partials of the weight gradients, kMaxSplits x max(out*in)
+  DevBuf cpart;  // column-sum partials of the bias gradients
 };
 
 namespace {
@@ -89,7 +92,14 @@ double loss_grad(esr_mlp* m, int64_t rows, const uint16_t* X, const float* T, hi
   for (int l = m->L - 1; l >= 0; --l) {
     const uint16_t* in = (l == 0) ? X : m->H[l - 1]->as<uint16_t>();
     const int32_t no = m->w[l + 1], ni = m->w[l];
-    // weight gradient dZ^T in   (no x ni), K = rows
+    // weight gradient dZ^T in   (no x ni), K = rows, split over K so the
+    // few output tiles still fill the chip; partials summed in fixed order
+    const int64_t tiles = (int64_t)((no + 63) / 64) * ((ni + 63) / 64);
+    int64_t ns = std::min<int64_t>({(int64_t)kMaxSplits, (512 + tiles - 1) / tiles, (rows + 255) / 256});
+    if (ns < 1) ns = 1;
+    int64_t kchunk = (rows + ns - 1) / ns;
+    kchunk = (kchunk + 31) / 32 * 32;
+    ns = (rows + kchunk - 1) / kchunk;
     GemmArgs g{};
     g.M = no;
     g.N = ni;
@@ -100,10 +110,18 @@ double loss_grad(esr_mlp* m, int64_t rows, const uint16_t* X, const float* T, hi
     g.B = in;
     g.sb_k = ni;
     g.sb_j = 1;
-    g.C32 = m->G.as<float>() + m->woff[l];
     g.ldc32 = ni;
-    gemm_bf16(g, st);
-    colsum(m->dA32.as<float>(), rows, no, no, m->G.as<float>() + m->boff[l], st);
+    if (ns > 1) {
+      g.C32 = m->split.as<float>();
+      g.kchunk = kchunk;
+      g.c_split = (int64_t)no * ni;
+      gemm_bf16(g, st);
+      sum_splits(m->split.as<float>(), (int32_t)ns, (int64_t)no * ni, m->G.as<float>() + m->woff[l], st);
+    } else {
+      g.C32 = m->G.as<float>() + m->woff[l];
+      gemm_bf16(g, st);
+    }
+    colsum(m->dA32.as<float>(), rows, no, no, m->cpart.as<float>(), m->G.as<float>() + m->boff[l], st);
     if (l > 0) {
       // data gradient dZ W   (rows x ni), K = no
       GemmArgs d{};
@@ -170,6 +188,10 @@ esr_status esr_mlp_create(int32_t n_layers, const int32_t* widths, int64_t max_r
     m->dA16.alloc((size_t)max_rows * m->maxw * sizeof(uint16_t));
     m->dB32.alloc((size_t)max_rows * m->maxw * sizeof(float));
     m->part.alloc(1024 * sizeof(double));
+    int64_t maxprod = 1;
+    for (int l = 0; l < n_layers; ++l) maxprod = std::max<int64_t>(maxprod, (int64_t)m->w[l] * m->w[l + 1]);
+    m->split.alloc((size_t)kMaxSplits * maxprod * sizeof(float));
+    m->cpart.alloc((size_t)256 * m->maxw * sizeof(float));
     hip_check(hipMemset(m->P.p, 0, off * sizeof(float)), "memset");
     hip_check(hipMemset(m->P16.p, 0, off * sizeof(uint16_t)), "memset");
     hip_check(hipMemset(m->M1.p, 0, off * sizeof(float)), "memset");

@@ -1,0 +1,262 @@
+"""Grid driver and warm-start harness on one GPU.
+
+* ``fixed_time``: the dataset runs of slurm/run_euler_grid.py:25-104 -- the
+  euler/main.py defaults, tf = 5 for the diamond and 3 otherwise
+  (eval/live_fvm.py:21-22), no early stop -- for a whole (Mach, alpha) grid in
+  one batched solver instead of one process per case.
+* ``training_pairs`` / ``train``: LR/HR pairs generated by the solver itself
+  and the warm-start network trained on them (warmstart.WarmNet).
+* ``WarmStarter.run``: the protocol of eval/convergence_case.py:30-142 --
+  for each case the residual threshold comes from the HR reference field
+  (x 1.001, see ``with_targets``), CD/CL targets are the reference field's,
+  stationarity is checked every 5 steps with patience 3
+  (eval/live_fvm.py:37-47) -- started cold (freestream), from the IDW
+  prediction, or from the network's prediction (LR solve -> features ->
+  network -> HR state), with the freestream fallback for a prediction with
+  rho <= 0 or p <= 0, or a solve that diverges. All cases of a call, of any
+  geometry, run in one batched solver.
+"""
+from __future__ import annotations
+
+import copy
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import main as M
+from . import warmstart as WS
+from .solve import BatchSolver
+
+REF_TF = {"diamond": 5.0}        # eval/live_fvm.py:21-22
+REF_TF_DEFAULT = 3.0
+CHECK_EVERY, PATIENCE = 5, 3     # eval/live_fvm.py:37-47
+THRESHOLD_MARGIN = 1.001         # eval/convergence_case.py:61-66
+ENGINEERING_TOL = 0.001          # eval/live_fvm.py:47
+
+
+def ref_tf(geometry: str) -> float:
+    return REF_TF.get(geometry, REF_TF_DEFAULT)
+
+
+def case_config(geometry, mach, aoa, **over):
+    """euler/main.py defaults for one (Mach, alpha) case."""
+    cfg = dict(rho_inf=1.0, p_inf=1.0, gamma=1.4, Mach=float(mach), aoa=float(aoa), case=geometry,
+               time_scheme="SRK2", flux="HLLC", reconstruction="MUSCL", CFL=0.6, tf=ref_tf(geometry))
+    cfg.update(over)
+    return cfg
+
+
+def make_cases(mesh, geometry, grid, **over):
+    specs, inits = [], []
+    for mach, aoa in grid:
+        cfg = case_config(geometry, mach, aoa, **over)
+        W, inlet = M.initialize(mesh, cfg)
+        sp = M.case_spec(mesh, inlet, cfg)
+        sp.tag = dict(geometry=geometry, mach=float(mach), aoa=float(aoa), h=mesh.metadata.get("h"))
+        specs.append(sp)
+        inits.append(W)
+    return specs, inits
+
+
+def fixed_time(specs, inits, max_steps=0):
+    """Runs every case to its N = int(tf/dt)+1 (no early stop). Returns the
+    reports (probe residual, CD, CL) and the final states (CUDA tensors)."""
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=CHECK_EVERY, patience=PATIENCE,
+                     max_steps=max_steps)
+    try:
+        for i, W in enumerate(inits):
+            bs.set_state(i, W)
+        bs.prepare()
+        bs.run()
+        bs.finalize()
+        reps = [bs.report(i) for i in range(len(specs))]
+        states = [bs.get_state(i) for i in range(len(specs))]
+    finally:
+        bs.close()
+    return reps, states
+
+
+def converge(specs, inits, max_steps=0, sync=True):
+    """Runs every case until its own stationarity criterion (spec.threshold,
+    check every 5, patience 3) or N; returns (reports, states, solve seconds)."""
+    import torch
+    bs = BatchSolver(specs, scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY, patience=PATIENCE,
+                     max_steps=max_steps)
+    try:
+        for i, W in enumerate(inits):
+            bs.set_state(i, W)
+        bs.prepare()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bs.run()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        bs.finalize()
+        reps = [bs.report(i) for i in range(len(specs))]
+        states = [bs.get_state(i) for i in range(len(specs))]
+    finally:
+        bs.close()
+    return reps, states, t
+
+
+def with_targets(specs, ref_reports, tf=1.0e4):
+    """Copies of the specs carrying the reference protocol: engineering
+    targets = reference CD/CL (0.1 %), tf large enough that stationarity, not
+    N, ends the solve, and threshold = 1.001 x the reference field's residual
+    (eval/convergence_case.py:61-66). The reference takes that residual from a
+    one-step probe and expects the reference field to pass its own check at
+    once; with the limiter's residual plateau a one-step probe can fall below
+    the 5-step check residual of the same field, so the residual used here is
+    the larger of the probe and the reference run's last `patience` check
+    residuals."""
+    out = []
+    for sp, r in zip(specs, ref_reports):
+        s = copy.copy(sp)
+        s.tf = float(tf)
+        base = r["final_residual"]
+        rm = r.get("recent_rel_max", float("nan"))
+        if rm == rm:
+            base = max(base, rm)
+        s.threshold = base * THRESHOLD_MARGIN
+        s.engineering = (r["cd"], r["cl"], ENGINEERING_TOL * max(abs(r["cd"]), 1e-8),
+                         ENGINEERING_TOL * max(abs(r["cl"]), 1e-8), PATIENCE)
+        out.append(s)
+    return out
+
+
+def training_pairs(geometry, lr_mesh, hr_mesh, grid, max_steps=0):
+    """Fixed-time LR and HR solves of every (Mach, alpha) in `grid` (one
+    batched solver for both resolutions). Returns a dict of CUDA tensors
+    lr (n, n_lr, 4), hr (n, n_hr, 4) and the grid."""
+    import torch
+    ls, li = make_cases(lr_mesh, geometry, grid)
+    hs, hi = make_cases(hr_mesh, geometry, grid)
+    reps, states = fixed_time(ls + hs, li + hi, max_steps=max_steps)
+    n = len(grid)
+    return dict(lr=torch.stack(states[:n]), hr=torch.stack(states[n:]), lr_reports=reps[:n], hr_reports=reps[n:],
+                machs=np.array([g[0] for g in grid]), aoas=np.array([g[1] for g in grid]))
+
+
+def primitive_stats(hr_states, gamma=1.4):
+    """Mean and standard deviation of the HR primitives over all cells of all
+    cases (the role of the dataset stats mu/sig, preprocessing/dataset.py)."""
+    P = WS.primitives(hr_states.reshape(-1, 4).cpu().numpy(), gamma)
+    return P.mean(0).astype(np.float32), np.maximum(P.std(0), 1e-6).astype(np.float32)
+
+
+def targets(kmap, pairs, mu, sig, gamma=1.4):
+    """Network inputs and targets T = (prim_HR - IDW) / sig on the GPU."""
+    import torch
+    X, idw = WS.features(kmap, pairs["lr"], pairs["machs"], pairs["aoas"], mu, sig, gamma)
+    P = WS.primitives(pairs["hr"].reshape(-1, 4).cpu().numpy(), gamma)
+    T = ((P - idw.cpu().numpy().astype(np.float64)) / sig.astype(np.float64)).astype(np.float32)
+    return X, idw, torch.from_numpy(T).to(X.device)
+
+
+def train(kmap, pairs, steps=3000, batch=16384, lr=1e-3, weight_decay=1e-5, seed=0, gamma=1.4, log_every=0):
+    """Trains a WarmNet on solver-generated pairs; returns (net, info)."""
+    import torch
+    mu, sig = primitive_stats(pairs["hr"], gamma)
+    X, idw, T = targets(kmap, pairs, mu, sig, gamma)
+    net = WS.WarmNet(in_dim=X.shape[1], max_rows=max(batch, 1 << 15), seed=seed)
+    net.mu, net.sig = mu, sig
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = net.fit(X, T, steps=steps, batch=batch, lr=lr, weight_decay=weight_decay, seed=seed,
+                   log_every=log_every)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    idw_mse = float(torch.mean(T.double() ** 2))
+    return net, dict(train_s=t, rows=int(X.shape[0]), steps=steps, batch=batch, loss_first=hist[0],
+                     loss_last=float(np.mean(hist[-20:])), idw_mse=idw_mse)
+
+
+@dataclass
+class WarmCase:
+    """One (geometry, Mach, alpha) case of the warm-start harness: its LR
+    fixed-time case, its HR case (threshold and targets set) and the HR
+    freestream state (cold start and fallback)."""
+    geometry: str
+    mach: float
+    aoa: float
+    lr_spec: object
+    lr_init: object
+    hr_spec: object
+    hr_init: object
+
+
+def warm_cases(geometry, lr_mesh, hr_mesh, grid, ref_reports):
+    """WarmCase list for `grid` on one mesh pair; ref_reports are the HR
+    reference fields' reports (threshold and CD/CL targets)."""
+    ls, li = make_cases(lr_mesh, geometry, grid)
+    hs, hi = make_cases(hr_mesh, geometry, grid)
+    hs = with_targets(hs, ref_reports)
+    return [WarmCase(geometry, float(m), float(a), ls[i], li[i], hs[i], hi[i]) for i, (m, a) in enumerate(grid)]
+
+
+class WarmStarter:
+    """LR solve -> features -> network -> HR warm start -> HR solve, batched
+    over cases of any number of geometries (one mesh pair and network each)."""
+
+    def __init__(self):
+        self.models = {}
+
+    def add(self, geometry, lr_mesh, hr_mesh, net=None, kmap=None):
+        self.models[geometry] = (kmap or WS.KnnMap(lr_mesh, hr_mesh), net)
+        return self
+
+    def predict(self, geometry, lr_states, grid, use_net=True, gamma=1.4):
+        """HR conserved states (n, n_hr, 4) and per-case fallback flags."""
+        import torch
+        kmap, net = self.models[geometry]
+        use = use_net and net is not None
+        mu, sig = (net.mu, net.sig) if use else (np.zeros(4, np.float32), np.ones(4, np.float32))
+        X, idw = WS.features(kmap, lr_states, [g[0] for g in grid], [g[1] for g in grid], mu, sig, gamma)
+        Y = net.forward(X) if use else None
+        W, bad = WS.assemble(kmap.n_hr, idw, Y, sig, gamma)
+        torch.cuda.synchronize()
+        return W.reshape(len(grid), kmap.n_hr, 4), bad.cpu().numpy().astype(bool)
+
+    def run(self, cases, mode="mlp", max_steps=0):
+        """One batched pass. mode: "cold" (freestream), "idw" or "mlp".
+        Returns per-case reports and the wall-time split (LR solve, inference,
+        HR solve, total)."""
+        import torch
+        n = len(cases)
+        out = dict(mode=mode, lr_s=0.0, infer_s=0.0, hr_s=0.0, fallback=[False] * n, retried=[])
+        torch.cuda.synchronize()
+        t_all = time.perf_counter()
+        inits = [c.hr_init for c in cases]
+        if mode != "cold":
+            t0 = time.perf_counter()
+            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps)
+            torch.cuda.synchronize()
+            out["lr_s"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for g in dict.fromkeys(c.geometry for c in cases):
+                ids = [i for i, c in enumerate(cases) if c.geometry == g]
+                W, bad = self.predict(g, torch.stack([lr_states[i] for i in ids]),
+                                      [(cases[i].mach, cases[i].aoa) for i in ids], use_net=(mode == "mlp"))
+                for j, i in enumerate(ids):
+                    out["fallback"][i] = bool(bad[j])
+                    if not bad[j]:
+                        inits[i] = W[j]
+            out["infer_s"] = time.perf_counter() - t0
+        reps, states, t = converge([c.hr_spec for c in cases], inits, max_steps=max_steps)
+        out["hr_s"] = t
+        # a warm start whose solve diverges is re-run from freestream
+        div = [i for i, r in enumerate(reps) if r["diverged"] and mode != "cold" and not out["fallback"][i]]
+        if div:
+            r2, s2, t2 = converge([cases[i].hr_spec for i in div], [cases[i].hr_init for i in div],
+                                  max_steps=max_steps)
+            out["hr_s"] += t2
+            for j, i in enumerate(div):
+                reps[i], states[i] = r2[j], s2[j]
+                out["fallback"][i] = True
+            out["retried"] = div
+        torch.cuda.synchronize()
+        out["total_s"] = time.perf_counter() - t_all
+        out["reports"] = reps
+        out["states"] = states
+        return out

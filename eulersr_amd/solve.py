@@ -112,7 +112,7 @@ class BatchSolver:
                     cd=float(r.cd), cl=float(r.cl),
                     engineering_converged=bool(r.engineering_converged),
                     engineering_stopping_step=int(r.engineering_stopping_step),
-                    diverged=bool(r.diverged))
+                    diverged=bool(r.diverged), recent_rel_max=float(r.recent_rel_max))
 
     def info(self):
         """(cell work-items, group cells, face work-items, group faces) of one stage."""
