@@ -163,8 +163,12 @@ def plain_single(cases, max_steps):
 def cd_drift(cases, stops, factor=2):
     """How settled the plain solve's CD is: CD of the cold-started case at
     `factor` x its stopping step (no early stop), read at the check there."""
+    import copy
     from eulersr_amd.solve import BatchSolver
-    bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3)
+    specs = [copy.copy(c.hr_spec) for c in cases]
+    for sp in specs:
+        sp.threshold = 0.0     # no early stop
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=3)
     out = [None] * len(cases)
     try:
         for i, c in enumerate(cases):

@@ -7,8 +7,8 @@
 last ``--tail`` dispatches (bench.py ends with back-to-back launches of each
 stage kernel, the same launches it times with HIP events).
 ``pmc``: per kernel, FETCH_SIZE and WRITE_SIZE (KiB per dispatch), and the
-same counters for the state-copy kernel k_copy, whose algorithmic traffic is
-known (16 B read + 16 B written per element), to give a calibration factor.
+same counters for a device copy of known size (bench.py --profile ends with
+two 1 GiB copies), to give a calibration factor.
 """
 from __future__ import annotations
 
@@ -41,7 +41,10 @@ def trace_summary(path, tail=50):
     return "\n".join(out)
 
 
-def pmc_summary(fetch_csv, write_csv, copy_bytes=None):
+def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_rocclr_copyBuffer"):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE means (KiB per dispatch). With
+    copy_bytes, the largest dispatch of `calib_kernel` (a device copy of
+    copy_bytes bytes each way) gives the factor from counter KiB to bytes."""
     def load(path, counter):
         vals = defaultdict(list)
         for r in csv.DictReader(open(path)):
@@ -56,11 +59,13 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes=None):
         fv = statistics.mean(f[k]) if f.get(k) else float("nan")
         wv = statistics.mean(w[k]) if w.get(k) else float("nan")
         out.append(f"{k:34s} {len(f.get(k, [])):7d} {fv:11.1f} {wv:11.1f}")
-    if copy_bytes and f.get("k_copy"):
-        fac_f = copy_bytes / 2 / 1024 / statistics.mean(f["k_copy"])
-        fac_w = copy_bytes / 2 / 1024 / statistics.mean(w["k_copy"])
-        out.append(f"k_copy calibration: algorithmic {copy_bytes / 2 / 1024:.1f} KiB each way -> "
-                   f"factor fetch {fac_f:.3f}, write {fac_w:.3f}")
+    ck = short(calib_kernel)
+    if copy_bytes and f.get(ck) and w.get(ck):
+        kib = copy_bytes / 1024
+        fac_f = kib / max(f[ck])
+        fac_w = kib / max(w[ck])
+        out.append(f"calibration ({ck}, largest dispatch = {kib:.0f} KiB each way): counter KiB x "
+                   f"{fac_f:.4f} (fetch), x {fac_w:.4f} (write) = bytes / 1024")
     return "\n".join(out)
 
 
@@ -74,11 +79,12 @@ def main():
     p.add_argument("fetch_csv")
     p.add_argument("write_csv")
     p.add_argument("--copy-bytes", type=float, default=None)
+    p.add_argument("--calib-kernel", default="__amd_rocclr_copyBuffer")
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace_summary(a.csv, a.tail))
     else:
-        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes))
+        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel))
 
 
 if __name__ == "__main__":
