@@ -69,6 +69,30 @@ def update_bytes(items, cells, f_items, faces):
     return f_items * 16 + items * 32 + cells * 20
 
 
+def cell_bytes(items, cells, f_items, faces):
+    """Compulsory bytes of one cell-centric flux + stage-1 update launch:
+    every limited face state read once (48 B per cell work-item; each is
+    used by its own cell and by the neighbour across the face), state in and
+    out (32 B); per group cell 3 face words (12 B), 3 normals (24 B), 3
+    lengths (12 B), 1/area and sponge weight (8 B)."""
+    return items * 80 + cells * 56
+
+
+KERNELS = (("k_recon", recon_bytes), ("k_face<MUSCL,HLLC>", face_bytes), ("k_update<stage1>", update_bytes),
+           ("k_cell<MUSCL,HLLC,stage1>", cell_bytes))
+# kernels of one stage on each path
+PATH_KERNELS = {"cell": ("k_recon", "k_cell<MUSCL,HLLC,stage1>"),
+                "face": ("k_recon", "k_face<MUSCL,HLLC>", "k_update<stage1>")}
+
+
+def kernel_table(sizes, ms):
+    kern = {}
+    for (name, fn), t in zip(KERNELS, ms):
+        b = fn(*sizes)
+        kern[name] = dict(us_per_launch=t * 1e3, bytes_per_launch=b, gbs=b / (t * 1e-3) / 1e9)
+    return kern
+
+
 TEST_MACHS, TEST_AOA = (0.9, 1.1, 2.0), 2.0
 # training grid of the warm-start network (the test cases are not in it)
 TRAIN_MACHS = (0.7, 0.8, 0.95, 1.05, 1.2, 1.4, 1.6, 1.8, 2.2, 2.5, 2.8)
@@ -112,12 +136,7 @@ def scale_measurement(meshes, n_cases, steps=200, reps=20):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     sizes = bs.info()
-    ms = bs.time_kernels(reps)
-    kern = {}
-    for name, fn, tk in (("k_recon", recon_bytes, ms[0]), ("k_face<MUSCL,HLLC>", face_bytes, ms[1]),
-                         ("k_update<stage1>", update_bytes, ms[2])):
-        b = fn(*sizes)
-        kern[name] = dict(us_per_launch=tk * 1e3, bytes_per_launch=b, gbs=b / (tk * 1e-3) / 1e9)
+    kern = kernel_table(sizes, bs.time_kernels(reps))
     bs.close()
     cells = sizes[0]
     return dict(cases=n_cases, cell_items=cells, us_per_step=t / steps * 1e6,
@@ -197,12 +216,7 @@ def stage_kernels(cases, reps):
         ms = bs.time_kernels(reps)
     finally:
         bs.close()
-    kern = {}
-    for name, fn, t in (("k_recon", recon_bytes, ms[0]), ("k_face<MUSCL,HLLC>", face_bytes, ms[1]),
-                        ("k_update<stage1>", update_bytes, ms[2])):
-        b = fn(*sizes)
-        kern[name] = dict(us_per_launch=t * 1e3, bytes_per_launch=b, gbs=b / (t * 1e-3) / 1e9)
-    return kern, sizes
+    return kernel_table(sizes, ms), sizes
 
 
 def main():
@@ -323,7 +337,9 @@ def main():
     for name, k in kern.items():
         log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.2f} MB -> "
             f"{k['gbs']:.0f} GB/s")
-    dom = max(kern, key=lambda k: kern[k]["us_per_launch"])
+    from eulersr_amd.solve import get_stage_path
+    path = get_stage_path()
+    dom = max(PATH_KERNELS[path], key=lambda k: kern[k]["us_per_launch"])
     scale = None
     if args.scale_cases > 0 and not args.profile:
         log(f"at-scale kernel measurement: {args.scale_cases} cases")
@@ -347,7 +363,7 @@ def main():
                    "cases": len(cases), "cells": cells, "h": args.h, "h_lr": args.h_lr},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": dom, "bytes_per_launch": kern[dom]["bytes_per_launch"],
+                     "kernel": dom, "stage_path": path, "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,
         "plain_single_case_cases_per_hour": single_rate,

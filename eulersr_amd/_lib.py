@@ -55,6 +55,9 @@ class SolveOpts(C.Structure):
     ]
 
 
+STAGE_AUTO, STAGE_CELL, STAGE_FACE = 0, 1, 2
+
+
 class KnnMap(C.Structure):
     _fields_ = [
         ("n_hr", C.c_int32), ("n_lr", C.c_int32), ("k", C.c_int32), ("knn_idx", C.c_void_p),
@@ -99,6 +102,8 @@ SIGNATURES = {
     "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
+    "esr_set_stage_path": (C.c_int, [C.c_int32]),
+    "esr_get_stage_path": (C.c_int32, []),
     "esr_mlp_create": (C.c_int, [C.c_int32, i32p, C.c_int64, C.POINTER(C.c_void_p)]),
     "esr_mlp_destroy": (C.c_int, [C.c_void_p]),
     "esr_mlp_param_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),

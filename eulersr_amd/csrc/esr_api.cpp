@@ -11,6 +11,7 @@
 // K steps and at step N with patience, states frozen once stopped, a final
 // one-step probe residual.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -90,9 +91,17 @@ struct CaseSpec {
 }  // namespace
 
 // Device state of a set of cases.
+// stage path of engines built from now on (esr_set_stage_path)
+std::atomic<int32_t> g_stage_path{ESR_STAGE_AUTO};
+int32_t resolved_stage_path() {
+  const int32_t p = g_stage_path.load();
+  return p == ESR_STAGE_FACE ? ESR_STAGE_FACE : ESR_STAGE_CELL;
+}
+
 struct Engine {
   int32_t n_cases = 0;
   int scheme = ESR_SCHEME_SRK2;
+  int32_t stage_path = ESR_STAGE_CELL;
   std::vector<GroupDev> groups;       // host copies
   std::vector<const esr_mesh*> gmesh; // mesh of each group
   std::vector<int32_t> case_group;    // global case -> group
@@ -125,6 +134,7 @@ struct Engine {
   // (flux, recon, entropy) share a launch.
   void build(const std::vector<CaseSpec>& specs, int scheme_) {
     scheme = scheme_;
+    stage_path = resolved_stage_path();
     (void)stages_of(scheme);
     n_cases = (int32_t)specs.size();
     if (n_cases <= 0) throw Error(ESR_ERR_VALUE, "no cases");
@@ -281,10 +291,14 @@ struct Engine {
         launch_zero_lf(n_cases, cases(), st);
         launch_lf_cmax(c->ls, cases(), Ws, rec.as<float4>(), st);
       }
-      launch_face(c->ls, cases(), Ws, rec.as<float4>(), fx.as<float4>(), st);
+      if (stage_path == ESR_STAGE_FACE) launch_face(c->ls, cases(), Ws, rec.as<float4>(), fx.as<float4>(), st);
     }
-    for (auto& c : classes)
-      launch_update(c->ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
+    for (auto& c : classes) {
+      if (stage_path == ESR_STAGE_FACE)
+        launch_update(c->ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
+      else
+        launch_cell(c->ls, mode, cases(), Ws, W0.as<float4>(), rec.as<float4>(), out, acc.as<float4>(), st);
+    }
   }
 
   // one Runge-Kutta step of every active case
@@ -381,6 +395,7 @@ Engine& mesh_engine(esr_mesh* m, const esr_opts& o, int scheme, hipStream_t st) 
   }
   Engine& e = *ws->eng;
   e.scheme = scheme;
+  e.stage_path = resolved_stage_path();
   return e;
 }
 
@@ -886,13 +901,23 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
   });
 }
 
+esr_status esr_set_stage_path(int32_t path) {
+  return guard([&] {
+    if (path != ESR_STAGE_AUTO && path != ESR_STAGE_CELL && path != ESR_STAGE_FACE)
+      throw Error(ESR_ERR_VALUE, "stage path must be ESR_STAGE_AUTO, ESR_STAGE_CELL or ESR_STAGE_FACE");
+    g_stage_path.store(path);
+  });
+}
+
+int32_t esr_get_stage_path(void) { return resolved_stage_path(); }
+
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream) {
   return guard([&] {
     if (!s || reps <= 0 || !ms) throw Error(ESR_ERR_VALUE, "bad arguments");
     hipStream_t st = S(stream);
     Engine& e = s->eng;
     launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
-    hipEvent_t ev[4];
+    hipEvent_t ev[5];
     for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
     auto recon = [&] {
       for (auto& cl : e.classes)
@@ -908,9 +933,15 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
         launch_update(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.fx.as<float4>(),
                       e.W1.as<float4>(), e.acc.as<float4>(), st);
     };
+    auto cell = [&] {
+      for (auto& cl : e.classes)
+        launch_cell(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
+                    e.W1.as<float4>(), e.acc.as<float4>(), st);
+    };
     recon();
     face();
-    update();  // code objects loaded, buffers valid
+    update();
+    cell();  // code objects loaded, buffers valid
     hip_check(hipEventRecord(ev[0], st), "record");
     for (int32_t r = 0; r < reps; ++r) recon();
     hip_check(hipEventRecord(ev[1], st), "record");
@@ -918,8 +949,10 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
     hip_check(hipEventRecord(ev[2], st), "record");
     for (int32_t r = 0; r < reps; ++r) update();
     hip_check(hipEventRecord(ev[3], st), "record");
-    hip_check(hipEventSynchronize(ev[3]), "event sync");
-    for (int q = 0; q < 3; ++q) {
+    for (int32_t r = 0; r < reps; ++r) cell();
+    hip_check(hipEventRecord(ev[4], st), "record");
+    hip_check(hipEventSynchronize(ev[4]), "event sync");
+    for (int q = 0; q < 4; ++q) {
       float t = 0.f;
       hip_check(hipEventElapsedTime(&t, ev[q], ev[q + 1]), "elapsed");
       ms[q] = (double)t / reps;

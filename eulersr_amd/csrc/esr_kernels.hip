@@ -625,27 +625,12 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
 // deterministic face-to-cell gather + residual + stage update
 // ---------------------------------------------------------------------------
 
-template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ groups,
-                                                  const int32_t* __restrict__ blk_group,
-                                                  const CaseDev* __restrict__ cases,
-                                                  const float4* __restrict__ Ws, const float4* W0,
-                                                  const float4* __restrict__ fx, float4* out, float4* acc) {
-  Item it;
-  if (!map_item<false>(groups, blk_group, cases, it)) return;
-  const GroupDev* G = it.G;
+// residual of one cell from its three face fluxes (F * len, outward), the
+// sponge source, and the stage update selected by `mode` (a StageMode)
+__device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell, int64_t s,
+                                               const float4 Wi, const float4 Fk[3], const float4* W0, float4* out,
+                                               float4* acc) {
   const MeshDev& m = G->m;
-  const int32_t n = m.n, cell = it.idx;
-  const CaseDev& cs = cases[G->case0 + it.c];
-
-  float4 Fk[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t cf = m.cface[(int64_t)k * n + cell];
-    const float4 f = fx[fidx_of(G, (int32_t)(cf & ~CFACE_NEG), it.c)];
-    Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
-  }
-  const float4 Wi = Ws[it.sidx];
   const float ia = m.inv_area[cell];
   float R[4];
   R[0] = ((Fk[0].x + Fk[1].x) + Fk[2].x) * ia;
@@ -659,31 +644,29 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
     R[2] = R[2] - sigma * (cs.value.z - Wi.z);
     R[3] = R[3] - sigma * (cs.value.w - Wi.w);
   }
-
-  const int64_t s = it.sidx;
-  if (MODE == MODE_RESID) {
+  if (mode == MODE_RESID) {
     out[s] = make_float4(R[0], R[1], R[2], R[3]);
     return;
   }
-  const float4 w0 = (MODE == MODE_EE) ? Wi : W0[s];
+  const float4 w0 = (mode == MODE_EE) ? Wi : W0[s];
   float4 o;
-  if (MODE == MODE_EE || MODE == MODE_S1) {
+  if (mode == MODE_EE || mode == MODE_S1) {
     const float dt = cs.dt;
     o = make_float4(w0.x - dt * R[0], w0.y - dt * R[1], w0.z - dt * R[2], w0.w - dt * R[3]);
-  } else if (MODE == MODE_SRK2_2) {
+  } else if (mode == MODE_SRK2_2) {
     const float dt = cs.dt;
     o = make_float4(0.5f * ((w0.x + Wi.x) - dt * R[0]), 0.5f * ((w0.y + Wi.y) - dt * R[1]),
                     0.5f * ((w0.z + Wi.z) - dt * R[2]), 0.5f * ((w0.w + Wi.w) - dt * R[3]));
-  } else if (MODE == MODE_RK2_2) {
+  } else if (mode == MODE_RK2_2) {
     const float dt = cs.dt;
     o = make_float4(0.5f * ((w0.x - dt * R[0]) + Wi.x), 0.5f * ((w0.y - dt * R[1]) + Wi.y),
                     0.5f * ((w0.z - dt * R[2]) + Wi.z), 0.5f * ((w0.w - dt * R[3]) + Wi.w));
-  } else if (MODE == MODE_RK4_1) {
+  } else if (mode == MODE_RK4_1) {
     const float h = cs.dt_half;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
     acc[s] = make_float4(R[0], R[1], R[2], R[3]);
-  } else if (MODE == MODE_RK4_2 || MODE == MODE_RK4_3) {
-    const float h = (MODE == MODE_RK4_2) ? cs.dt_half : cs.dt;
+  } else if (mode == MODE_RK4_2 || mode == MODE_RK4_3) {
+    const float h = (mode == MODE_RK4_2) ? cs.dt_half : cs.dt;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
     const float4 a = acc[s];
     acc[s] = make_float4(a.x + 2.0f * R[0], a.y + 2.0f * R[1], a.z + 2.0f * R[2], a.w + 2.0f * R[3]);
@@ -694,6 +677,83 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
                     w0.w - h * (a.w + R[3]));
   }
   out[s] = o;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ groups,
+                                                  const int32_t* __restrict__ blk_group,
+                                                  const CaseDev* __restrict__ cases,
+                                                  const float4* __restrict__ Ws, const float4* W0,
+                                                  const float4* __restrict__ fx, float4* out, float4* acc) {
+  Item it;
+  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const int32_t n = m.n, cell = it.idx;
+  const CaseDev& cs = cases[G->case0 + it.c];
+  float4 Fk[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t cf = m.cface[(int64_t)k * n + cell];
+    const float4 f = fx[fidx_of(G, (int32_t)(cf & ~CFACE_NEG), it.c)];
+    Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
+  }
+  stage_epilogue(MODE, G, cs, cell, it.sidx, Ws[it.sidx], Fk, W0, out, acc);
+}
+
+// ---------------------------------------------------------------------------
+// cell-centric stage: the three face fluxes of a cell from its own and its
+// neighbours' face states (each interior face is evaluated by both of its
+// cells, as the reference's per-cell flux arrays do), then the residual and
+// the stage update -- one launch instead of face + update.
+// ---------------------------------------------------------------------------
+
+template <int RECON, int FLUX, int ENT>
+__global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ groups,
+                                                const int32_t* __restrict__ blk_group,
+                                                const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
+                                                const float4* W0, const float4* __restrict__ rec, float4* out,
+                                                float4* acc, int mode) {
+  Item it;
+  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  const GroupDev* G = it.G;
+  const MeshDev& m = G->m;
+  const Phys& ph = G->ph;
+  const int32_t n = m.n, cell = it.idx;
+  const CaseDev& cs = cases[G->case0 + it.c];
+  const float4 Wi = Ws[it.sidx];
+  const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
+  float4 Fk[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint32_t f = m.fref[(int64_t)k * n + cell];
+    const float2 nk = m.nrm[(int64_t)k * n + cell];
+    const float sk = m.len[(int64_t)k * n + cell];
+    float4 WL, WR;
+    if (fref_is_bnd(f)) {
+      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
+      const float2 nw = (kind == G_WALL) ? nk : make_float2(0.f, 0.f);
+      const float4 Wg = ghost_state(kind, Wi, nw, cs.value, m.inlet_subsonic, Ws,
+                                    sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+      if (RECON == ESR_RECON_MUSCL) {
+        WL = cons_of(as_prim(rec[ridx_of(G, k, cell, it.c)]), ph.inv_gm1M2);
+        WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.inv_gm1M2);
+      } else {
+        WL = Wi;
+        WR = Wg;
+      }
+    } else if (RECON == ESR_RECON_MUSCL) {
+      WL = cons_of(as_prim(rec[ridx_of(G, k, cell, it.c)]), ph.inv_gm1M2);
+      WR = cons_of(as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)]), ph.inv_gm1M2);
+    } else {
+      WL = Wi;
+      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+    }
+    float F[4];
+    face_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, ph, lf_c, F);
+    Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
+  }
+  stage_epilogue(mode, G, cs, cell, it.sidx, Wi, Fk, W0, out, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1088,6 +1148,45 @@ void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, co
       launch_face_f<ESR_RECON_CONSTANT, 0>(ls, cases, Ws, rec, fx, st);
   }
   check_launch("k_face");
+}
+
+template <int RECON, int ENT>
+static void launch_cell_f(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                          const float4* rec, float4* out, float4* acc, hipStream_t st) {
+  const dim3 g(ls.nblocks), b(BLOCK);
+#define ESR_CELL_CASE(FL)                                                                                    \
+  case FL:                                                                                                   \
+    hipLaunchKernelGGL((k_cell<RECON, FL, ENT>), g, b, 0, st, ls.groups, ls.blk_group, cases, Ws, W0, rec, out, \
+                       acc, mode);                                                                           \
+    break;
+  switch (ls.flux) {
+    ESR_CELL_CASE(ESR_FLUX_LF)
+    ESR_CELL_CASE(ESR_FLUX_RUSANOV)
+    ESR_CELL_CASE(ESR_FLUX_TADMOR)
+    ESR_CELL_CASE(ESR_FLUX_AUSM)
+    ESR_CELL_CASE(ESR_FLUX_HLLC)
+    ESR_CELL_CASE(ESR_FLUX_ROE)
+    default: throw Error(ESR_ERR_VALUE, "unknown flux");
+  }
+#undef ESR_CELL_CASE
+}
+
+void launch_cell(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                 const float4* rec, float4* out, float4* acc, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  if (mode < MODE_RESID || mode > MODE_RK4_4) throw Error(ESR_ERR_VALUE, "unknown stage mode");
+  if (ls.recon == ESR_RECON_MUSCL) {
+    if (ls.entropy)
+      launch_cell_f<ESR_RECON_MUSCL, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+    else
+      launch_cell_f<ESR_RECON_MUSCL, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+  } else {
+    if (ls.entropy)
+      launch_cell_f<ESR_RECON_CONSTANT, 1>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+    else
+      launch_cell_f<ESR_RECON_CONSTANT, 0>(ls, mode, cases, Ws, W0, rec, out, acc, st);
+  }
+  check_launch("k_cell");
 }
 
 void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,

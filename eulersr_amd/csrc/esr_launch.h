@@ -27,6 +27,10 @@ void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, co
                  hipStream_t st);
 void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
                    const float4* fx, float4* out, float4* acc, hipStream_t st);
+// cell-centric stage (face fluxes of each cell + residual + update in one
+// launch); the alternative to launch_face + launch_update
+void launch_cell(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                 const float4* rec, float4* out, float4* acc, hipStream_t st);
 
 // Per-case reductions over one group (host-side group descriptor `g`,
 // device copy at `gdev`). Partial results at part[case * rstride + r].

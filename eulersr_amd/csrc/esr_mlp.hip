@@ -124,26 +124,46 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
       }
 }
 
-// out[e] = sum over s of part[s*n + e], s ascending
+// out[e] = sum over s of part[s*n + e]: four interleaved partial sums
+// (s mod 4) combined in a fixed order
 __global__ void k_sum_splits(const float* __restrict__ part, int32_t ns, int64_t n, float* __restrict__ out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  float s = 0.f;
-  for (int32_t q = 0; q < ns; ++q) s += part[q * n + e];
-  out[e] = s;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+  int32_t q = 0;
+  for (; q + 4 <= ns; q += 4) {
+    a0 += part[(int64_t)q * n + e];
+    a1 += part[(int64_t)(q + 1) * n + e];
+    a2 += part[(int64_t)(q + 2) * n + e];
+    a3 += part[(int64_t)(q + 3) * n + e];
+  }
+  for (; q < ns; ++q) a0 += part[(int64_t)q * n + e];
+  out[e] = (a0 + a1) + (a2 + a3);
 }
 
-// partial column sums of an fp32 (rows x n) matrix: block (x, y) sums rows
-// [y*chunk, (y+1)*chunk) of 64 columns into part[y*n + j], rows ascending
-__global__ void k_colsum_part(const float* __restrict__ X, int64_t rows, int32_t n, int64_t ld, int64_t chunk,
-                              float* __restrict__ part) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+// partial column sums of an fp32 (rows x n) matrix: block (x, y) of 256
+// threads covers 64 columns and rows [y*chunk, (y+1)*chunk); thread row-lane
+// t = tid/64 sums rows r0 + t, r0 + t + 4, ... with two accumulators; the
+// four lanes are combined in a fixed order into part[y*n + j]
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X, int64_t rows, int32_t n,
+                                                     int64_t ld, int64_t chunk, float* __restrict__ part) {
+  __shared__ float sh[4][64];
+  const int c = threadIdx.x & 63, t = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + c;
   const int64_t r0 = (int64_t)blockIdx.y * chunk;
   const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
-  float s = 0.f;
-  for (int64_t i = r0; i < r1; ++i) s += X[i * ld + j];
-  part[(int64_t)blockIdx.y * n + j] = s;
+  float a0 = 0.f, a1 = 0.f;
+  if (j < n) {
+    int64_t i = r0 + t;
+    for (; i + 4 < r1; i += 8) {
+      a0 += X[i * ld + j];
+      a1 += X[(i + 4) * ld + j];
+    }
+    for (; i < r1; i += 4) a0 += X[i * ld + j];
+  }
+  sh[t][c] = a0 + a1;
+  __syncthreads();
+  if (t == 0 && j < n) part[(int64_t)blockIdx.y * n + j] = (sh[0][c] + sh[1][c]) + (sh[2][c] + sh[3][c]);
 }
 
 // MSE of (rows x n) predictions against targets; dY = 2 (Y - T) / (rows n)
@@ -210,6 +230,10 @@ __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y,
 // wall features (4), the normalised Mach and angle of attack (2); zero
 // padding up to in_pad. Also the IDW prediction sum_j w_j prim_j
 // (eval/core.py:135-142).
+// Row assembly with the neighbour count K known at compile time: the row is
+// built in registers and written as 16-byte stores (in_pad % 8 == 0, the
+// row start is then 16-byte aligned). K = 0: any k, element stores.
+template <int K>
 __global__ void k_warm_features(int32_t n_hr, int32_t n_lr, int32_t k, const int32_t* __restrict__ idx,
                                 const float* __restrict__ rel, const float* __restrict__ w,
                                 const float* __restrict__ wall, int32_t n_cases, const float* __restrict__ lrW,
@@ -218,31 +242,60 @@ __global__ void k_warm_features(int32_t n_hr, int32_t n_lr, int32_t k, const int
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (row >= (int64_t)n_hr * n_cases) return;
   const int32_t c = (int32_t)(row / n_hr), i = (int32_t)(row - (int64_t)c * n_hr);
+  if (K > 0) k = K;
   uint16_t* x = X + row * in_pad;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  constexpr int NV = K > 0 ? ((7 * K + 6 + 7) / 8) * 8 : 8;
+  float v[NV];
   int col = 0;
   for (int j = 0; j < k; ++j) {
     const int32_t l = idx[(int64_t)i * k + j];
     const float4 W = reinterpret_cast<const float4*>(lrW)[(int64_t)c * n_lr + l];
-    const float r = W.x, u = W.y / r, v = W.z / r;
-    const float P = (W.w - 0.5f * r * (u * u + v * v)) * gm1;
+    const float r = W.x, u = W.y / r, vv = W.z / r;
+    const float P = (W.w - 0.5f * r * (u * u + vv * vv)) * gm1;
     const float wj = w[(int64_t)i * k + j];
     acc.x += wj * r;
     acc.y += wj * u;
-    acc.z += wj * v;
+    acc.z += wj * vv;
     acc.w += wj * P;
-    x[col++] = f2bf((r - mu.x) * isig.x);
-    x[col++] = f2bf((u - mu.y) * isig.y);
-    x[col++] = f2bf((v - mu.z) * isig.z);
-    x[col++] = f2bf((P - mu.w) * isig.w);
-    x[col++] = f2bf(rel[((int64_t)i * k + j) * 2]);
-    x[col++] = f2bf(rel[((int64_t)i * k + j) * 2 + 1]);
-    x[col++] = f2bf(wj);
+    const float2 rl = reinterpret_cast<const float2*>(rel)[(int64_t)i * k + j];
+    const float f[7] = {(r - mu.x) * isig.x, (u - mu.y) * isig.y, (vv - mu.z) * isig.z, (P - mu.w) * isig.w,
+                        rl.x, rl.y, wj};
+    if (K > 0) {
+#pragma unroll
+      for (int q = 0; q < 7; ++q) v[7 * j + q] = f[q];
+    } else {
+      for (int q = 0; q < 7; ++q) x[col + q] = f2bf(f[q]);
+      col += 7;
+    }
   }
-  for (int q = 0; q < 4; ++q) x[col++] = f2bf(wall[(int64_t)i * 4 + q]);
-  x[col++] = f2bf(scal[2 * c]);
-  x[col++] = f2bf(scal[2 * c + 1]);
-  for (; col < in_pad; ++col) x[col] = 0;
+  const float4 wf = reinterpret_cast<const float4*>(wall)[i];
+  const float tail[6] = {wf.x, wf.y, wf.z, wf.w, scal[2 * c], scal[2 * c + 1]};
+  if (K > 0) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) v[7 * K + q] = tail[q];
+#pragma unroll
+    for (int q = 7 * K + 6; q < NV; ++q) v[q] = 0.f;
+    if ((in_pad & 7) == 0 && in_pad >= NV) {
+#pragma unroll
+      for (int b = 0; b < NV / 8; ++b) {
+        uint4 u4;
+        u4.x = f2bf(v[8 * b + 0]) | ((uint32_t)f2bf(v[8 * b + 1]) << 16);
+        u4.y = f2bf(v[8 * b + 2]) | ((uint32_t)f2bf(v[8 * b + 3]) << 16);
+        u4.z = f2bf(v[8 * b + 4]) | ((uint32_t)f2bf(v[8 * b + 5]) << 16);
+        u4.w = f2bf(v[8 * b + 6]) | ((uint32_t)f2bf(v[8 * b + 7]) << 16);
+        reinterpret_cast<uint4*>(x)[b] = u4;
+      }
+      for (int q = NV; q < in_pad; ++q) x[q] = 0;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 7 * K + 6; ++q) x[q] = f2bf(v[q]);
+      for (int q = 7 * K + 6; q < in_pad; ++q) x[q] = 0;
+    }
+  } else {
+    for (int q = 0; q < 6; ++q) x[col + q] = f2bf(tail[q]);
+    for (col += 6; col < in_pad; ++col) x[col] = 0;
+  }
   reinterpret_cast<float4*>(idw)[row] = acc;
 }
 
@@ -299,14 +352,14 @@ void sum_splits(const float* part, int32_t ns, int64_t n, float* out, hipStream_
 }
 
 int32_t colsum_splits(int64_t rows) {
-  const int64_t s = (rows + 1023) / 1024;
+  const int64_t s = (rows + 255) / 256;
   return (int32_t)(s < 1 ? 1 : (s > 256 ? 256 : s));
 }
 
 void colsum(const float* X, int64_t rows, int32_t n, int64_t ld, float* part, float* out, hipStream_t st) {
   const int32_t ns = colsum_splits(rows);
   const int64_t chunk = (rows + ns - 1) / ns;
-  hipLaunchKernelGGL(k_colsum_part, dim3((n + 63) / 64, ns), dim3(64), 0, st, X, rows, n, ld, chunk, part);
+  hipLaunchKernelGGL(k_colsum_part, dim3((n + 63) / 64, ns), dim3(256), 0, st, X, rows, n, ld, chunk, part);
   hip_check(hipGetLastError(), "k_colsum_part");
   sum_splits(part, ns, n, out, st);
 }
@@ -343,8 +396,13 @@ void warm_features(int32_t n_hr, int32_t n_lr, int32_t k, const int32_t* idx, co
                    const float* wall, int32_t n_cases, const float* lrW, const float* scal, float4 mu, float4 isig,
                    float gm1, int32_t in_pad, uint16_t* X, float* idw, hipStream_t st) {
   const int64_t rows = (int64_t)n_hr * n_cases;
-  hipLaunchKernelGGL(k_warm_features, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, n_hr, n_lr, k, idx, rel,
-                     w, wall, n_cases, lrW, scal, mu, isig, gm1, in_pad, X, idw);
+  const dim3 grid((unsigned)((rows + 255) / 256));
+  if (k == 6)
+    hipLaunchKernelGGL(k_warm_features<6>, grid, dim3(256), 0, st, n_hr, n_lr, k, idx, rel, w, wall, n_cases, lrW,
+                       scal, mu, isig, gm1, in_pad, X, idw);
+  else
+    hipLaunchKernelGGL(k_warm_features<0>, grid, dim3(256), 0, st, n_hr, n_lr, k, idx, rel, w, wall, n_cases, lrW,
+                       scal, mu, isig, gm1, in_pad, X, idw);
   hip_check(hipGetLastError(), "k_warm_features");
 }
 

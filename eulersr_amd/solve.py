@@ -36,6 +36,17 @@ def transonic_cfl(mach: float, cfl: float) -> float:
     return 0.45 if (0.6 < mach < 1.1 and cfl > 0.45) else cfl
 
 
+def set_stage_path(path: str):
+    """"cell" (default), "face" or "auto": the stage kernels of solvers and
+    per-mesh engines built afterwards."""
+    code = {"auto": L.STAGE_AUTO, "cell": L.STAGE_CELL, "face": L.STAGE_FACE}[path]
+    L.check(L.lib().esr_set_stage_path(code), "esr_set_stage_path")
+
+
+def get_stage_path() -> str:
+    return {L.STAGE_CELL: "cell", L.STAGE_FACE: "face"}[L.lib().esr_get_stage_path()]
+
+
 class BatchSolver:
     def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
                  graph=True):
@@ -122,8 +133,9 @@ class BatchSolver:
 
     def time_kernels(self, reps=20):
         """Average ms per launch of the (reconstruction, face flux, gather +
-        update) kernels, HIP events on the current stream."""
-        ms = (C.c_double * 3)()
+        update, cell-centric flux + update) kernels, HIP events on the current
+        stream."""
+        ms = (C.c_double * 4)()
         L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), ms, stream_ptr()), "esr_solver_time_kernels")
         return tuple(ms)
 

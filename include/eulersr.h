@@ -244,10 +244,24 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
                            int64_t* group_faces);
 /* Average device time (ms) of one launch of each stage kernel over the
  * solver's current states: ms[0] reconstruction, ms[1] face flux, ms[2]
- * gather + stage-1 update. Each is launched `reps` times back to back
- * between two HIP events on `stream`; all three are idempotent (they
- * rewrite scratch buffers and the stage buffer only). */
+ * gather + stage-1 update (the per-face path), ms[3] cell-centric flux +
+ * stage-1 update (the cell path). Each is launched `reps` times back to back
+ * between two HIP events on `stream`; all are idempotent (they rewrite
+ * scratch buffers and the stage buffer only). ms holds 4 entries. */
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
+
+/* Stage path of the engines built after the call (solvers created, and the
+ * per-mesh engine of esr_residual / esr_step at its next call):
+ * ESR_STAGE_CELL -- per stage a reconstruction launch and one cell-centric
+ * launch that evaluates the three face fluxes of each cell (each interior
+ * face from both of its cells, like the reference's per-cell flux arrays),
+ * the residual and the update; ESR_STAGE_FACE -- reconstruction, one flux per
+ * logical face, then a gather + update launch (each face evaluated once, the
+ * neighbour takes its negation). ESR_STAGE_AUTO is the cell path. No
+ * counterpart in the reference (a tuning choice of this implementation). */
+enum esr_stage_path { ESR_STAGE_AUTO = 0, ESR_STAGE_CELL = 1, ESR_STAGE_FACE = 2 };
+esr_status esr_set_stage_path(int32_t path);
+int32_t esr_get_stage_path(void);
 const char* esr_last_error(void);
 int32_t esr_version(void);
 

@@ -61,3 +61,14 @@ def test_option_errors_match_reference():
     o = make_opts(dict(flux="HLL", reconstruction="second_order"))
     from eulersr_amd import _lib
     assert o.flux == _lib.FLUX_HLLC and o.recon == _lib.RECON_MUSCL
+
+
+def test_stage_path_setting_is_host_only_and_validated():
+    from eulersr_amd import _lib
+    from eulersr_amd.solve import get_stage_path, set_stage_path
+    L = _lib.lib()
+    assert L.esr_set_stage_path(7) == 1          # ESR_ERR_VALUE
+    assert b"stage path" in L.esr_last_error()
+    for p in ("face", "cell", "auto"):
+        set_stage_path(p)
+        assert get_stage_path() == ("cell" if p == "auto" else p)

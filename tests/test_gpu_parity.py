@@ -15,6 +15,17 @@ from conftest import base_cfg, mesh_of, perturbed_state
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=["cell", "face"])
+def stage_path(request):
+    """Every parity test runs on both stage paths: cell-centric fluxes fused
+    with the update, and per-face fluxes followed by a gather + update."""
+    from eulersr_amd.solve import get_stage_path, set_stage_path
+    set_stage_path(request.param)
+    assert get_stage_path() == request.param
+    yield request.param
+    set_stage_path("auto")
+
 FLUXES = ["hllc", "roe", "rusanov", "lf", "ausm", "tadmor"]
 RECONS = ["constant", "muscl"]
 

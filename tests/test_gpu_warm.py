@@ -129,21 +129,25 @@ def _lr_states(lr_mesh, n_cases, seed):
                     -1).astype(np.float32)
 
 
-def test_features_and_apply_match_oracle(pair):
+@pytest.mark.parametrize("k,in_pad", [(6, None), (6, 64), (4, None), (4, 40)])
+def test_features_and_apply_match_oracle(pair, k, in_pad):
     torch = _dev()
     from oracle import mlp as OM
     from oracle import warm as OW
     from eulersr_amd import warmstart as WS
+    print(f"features k={k} in_pad={in_pad}", flush=True)
     lr, hr = pair
-    km = WS.KnnMap(lr, hr)
+    km = WS.KnnMap(lr, hr, k=k)
+    width = in_pad or WS.n_features(k)
     n_cases = 3
     W = _lr_states(lr, n_cases, 0)
     machs, aoas = [0.8, 1.5, 2.5], [-2.0, 0.0, 4.5]
     mu = np.array([1.1, 0.9, 0.05, 1.2], np.float32)
     sig = np.array([0.4, 0.5, 0.3, 0.6], np.float32)
-    X, idw = WS.features(km, torch.from_numpy(W).cuda(), machs, aoas, mu, sig)
+    X, idw = WS.features(km, torch.from_numpy(W).cuda(), machs, aoas, mu, sig, in_pad=in_pad)
+    assert X.shape == (n_cases * km.n_hr, width)
     Xg = X.float().cpu().numpy()
-    Xo, idw_o = OW.features(km.idx, km.rel, km.w, km.wall, W, WS.case_scalars(machs, aoas), mu, sig, 1.4, 48)
+    Xo, idw_o = OW.features(km.idx, km.rel, km.w, km.wall, W, WS.case_scalars(machs, aoas), mu, sig, 1.4, width)
     Xo16 = OM.bf16(Xo)
     exact = (Xg == Xo16)
     assert exact.mean() > 0.999, exact.mean()
