@@ -125,7 +125,7 @@ def scale_measurement(meshes, n_cases, steps=200, reps=20):
         W, inlet = M.initialize(meshes[g], cfg)
         specs.append(M.case_spec(meshes[g], inlet, cfg))
         inits.append(W)
-    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=3)
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=3, autotune=True)
     for i, W in enumerate(inits):
         bs.set_state(i, W)
     bs.prepare()
@@ -137,10 +137,11 @@ def scale_measurement(meshes, n_cases, steps=200, reps=20):
     t = time.perf_counter() - t0
     sizes = bs.info()
     kern = kernel_table(sizes, bs.time_kernels(reps))
+    tuning = bs.tuning
     bs.close()
     cells = sizes[0]
     return dict(cases=n_cases, cell_items=cells, us_per_step=t / steps * 1e6,
-                cell_updates_per_s=cells * steps / t, stage_kernels=kern)
+                cell_updates_per_s=cells * steps / t, stage_kernels=kern, tuning=tuning)
 
 
 def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
@@ -206,7 +207,8 @@ def stage_kernels(cases, reps):
     """Live HIP-event timing of the stage kernels on a batched solver of the
     given HR cases (freestream states advanced 10 steps)."""
     from eulersr_amd.solve import BatchSolver
-    bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3)
+    bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3,
+                     autotune=True)
     try:
         for i, c in enumerate(cases):
             bs.set_state(i, c.hr_init)
@@ -214,9 +216,10 @@ def stage_kernels(cases, reps):
         bs.run(10)
         sizes = bs.info()
         ms = bs.time_kernels(reps)
+        tuning = bs.tuning
     finally:
         bs.close()
-    return kernel_table(sizes, ms), sizes
+    return kernel_table(sizes, ms), sizes, tuning
 
 
 def main():
@@ -333,12 +336,12 @@ def main():
         drift = [abs(a - p["cd"]) for a, p in zip(later, single)]
         log(f"plain CD change from its stopping step to 2x that step: {['%.1e' % d for d in drift]}")
 
-    kern, sizes = stage_kernels(cases, args.kernel_reps)
+    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps)
+    log(f"autotuned launch of the 6-case batch: {tuning}")
     for name, k in kern.items():
         log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.2f} MB -> "
             f"{k['gbs']:.0f} GB/s")
-    from eulersr_amd.solve import get_stage_path
-    path = get_stage_path()
+    path = tuning["path"]
     dom = max(PATH_KERNELS[path], key=lambda k: kern[k]["us_per_launch"])
     scale = None
     if args.scale_cases > 0 and not args.profile:
@@ -346,7 +349,8 @@ def main():
         scale = scale_measurement({g: hm for g, (lm, hm) in meshes.items()}, args.scale_cases)
         for k, v in scale["stage_kernels"].items():
             log(f"  {k}: {v['us_per_launch']:.1f} us/launch -> {v['gbs']:.0f} GB/s")
-        log(f"  {scale['us_per_step']:.1f} us/step, {scale['cell_updates_per_s']:.3g} cell-updates/s")
+        log(f"  {scale['us_per_step']:.1f} us/step, {scale['cell_updates_per_s']:.3g} cell-updates/s, "
+            f"autotuned {scale['tuning']}")
 
     steps_case = [x["stopping_step"] if x["converged"] else x["steps_run"] for x in warm["reports"]]
     cells = [c.hr_spec.mesh.n_cells for c in cases]
@@ -383,6 +387,7 @@ def main():
         "thresholds": [c.hr_spec.threshold for c in cases],
         "training": training,
         "stage_kernels": kern,
+        "tuning": tuning,
         "at_scale": scale,
         "launch_sizes": dict(zip(("cell_items", "group_cells", "face_items", "group_faces"), sizes)),
     }

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <set>
 #include <tuple>
 
@@ -91,6 +92,14 @@ struct CaseSpec {
 }  // namespace
 
 // Device state of a set of cases.
+// autotuned launch choices by launch shape (esr_solver_autotune)
+struct TuneChoice {
+  int32_t path = ESR_STAGE_CELL, block = 256;
+  double ms = 0.0;
+};
+std::mutex g_tune_mu;
+std::map<std::vector<int64_t>, TuneChoice> g_tune;
+
 // stage path of engines built from now on (esr_set_stage_path)
 std::atomic<int32_t> g_stage_path{ESR_STAGE_AUTO};
 int32_t resolved_stage_path() {
@@ -111,6 +120,7 @@ struct Engine {
   struct Cls {
     LaunchSet ls;
     DevBuf blk, fblk;
+    std::vector<int32_t> gids;  // groups of the class
   };
   std::vector<std::unique_ptr<Cls>> classes;
   bool any_muscl = false, any_lf = false;
@@ -123,6 +133,39 @@ struct Engine {
 
   ~Engine() {
     if (count_h) (void)hipHostFree(count_h);
+  }
+
+  // cell / face block tables of a class for `block` threads per block
+  // (host group copies updated; the caller uploads groups_d)
+  void block_tables(Cls& c, int32_t block) {
+    std::vector<int32_t> blk, fblk;
+    for (int32_t gi : c.gids) {
+      GroupDev& g = groups[gi];
+      const int64_t items = (int64_t)g.m.n * g.C;
+      g.blk0 = (int32_t)blk.size();
+      g.nblk = (int32_t)((items + block - 1) / block);
+      for (int32_t b = 0; b < g.nblk; ++b) blk.push_back(gi);
+      const int64_t fitems = (int64_t)g.m.nf * g.C;
+      g.fblk0 = (int32_t)fblk.size();
+      g.fnblk = (int32_t)((fitems + block - 1) / block);
+      for (int32_t b = 0; b < g.fnblk; ++b) fblk.push_back(gi);
+    }
+    upload(c.blk, blk);
+    upload(c.fblk, fblk);
+    c.ls.blk_group = c.blk.as<int32_t>();
+    c.ls.nblocks = (int32_t)blk.size();
+    c.ls.fblk_group = c.fblk.as<int32_t>();
+    c.ls.fnblocks = (int32_t)fblk.size();
+    c.ls.block = block;
+  }
+
+  // one launch configuration for every class: stage path and block size
+  void configure(int32_t path, int32_t block, hipStream_t st) {
+    hip_check(hipStreamSynchronize(st), "configure sync");
+    for (auto& c : classes) block_tables(*c, block);
+    hip_check(hipMemcpy(groups_d.p, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice),
+              "groups H2D");
+    stage_path = path;
   }
 
   float4* buf(int which) {
@@ -204,27 +247,11 @@ struct Engine {
     }
     for (auto& kv : cls) {
       auto c = std::make_unique<Cls>();
-      std::vector<int32_t> blk, fblk;
-      for (int32_t gi : kv.second) {
-        GroupDev& g = groups[gi];
-        const int64_t items = (int64_t)g.m.n * g.C;
-        g.blk0 = (int32_t)blk.size();
-        g.nblk = (int32_t)((items + BLOCK - 1) / BLOCK);
-        for (int32_t b = 0; b < g.nblk; ++b) blk.push_back(gi);
-        const int64_t fitems = (int64_t)g.m.nf * g.C;
-        g.fblk0 = (int32_t)fblk.size();
-        g.fnblk = (int32_t)((fitems + BLOCK - 1) / BLOCK);
-        for (int32_t b = 0; b < g.fnblk; ++b) fblk.push_back(gi);
-      }
-      upload(c->blk, blk);
-      upload(c->fblk, fblk);
-      c->ls.blk_group = c->blk.as<int32_t>();
-      c->ls.nblocks = (int32_t)blk.size();
-      c->ls.fblk_group = c->fblk.as<int32_t>();
-      c->ls.fnblocks = (int32_t)fblk.size();
+      c->gids = kv.second;
       c->ls.flux = std::get<0>(kv.first);
       c->ls.recon = std::get<1>(kv.first);
       c->ls.entropy = std::get<2>(kv.first);
+      block_tables(*c, BLOCK);
       classes.push_back(std::move(c));
     }
     upload(groups_d, groups);
@@ -898,6 +925,85 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
     if (group_cells) *group_cells = cells;
     if (face_items) *face_items = fitems;
     if (group_faces) *group_faces = faces;
+  });
+}
+
+esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, double* us_per_stage,
+                               void* stream) {
+  return guard([&] {
+    if (!s || reps < 1) throw Error(ESR_ERR_VALUE, "bad arguments");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
+    // launch shape of the solver: per class its kernel configuration and sizes
+    std::vector<int64_t> key{(int64_t)e.scheme};
+    for (auto& c : e.classes) {
+      int64_t items = 0, fitems = 0;
+      for (int32_t gi : c->gids) {
+        items += (int64_t)e.groups[gi].m.n * e.groups[gi].C;
+        fitems += (int64_t)e.groups[gi].m.nf * e.groups[gi].C;
+      }
+      key.insert(key.end(), {c->ls.flux, c->ls.recon, c->ls.entropy, (int64_t)c->gids.size(), items, fitems});
+    }
+    TuneChoice best{};
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tune.find(key);
+      if (it != g_tune.end()) {
+        best = it->second;
+        cached = true;
+      }
+    }
+    if (!cached) {
+      launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
+      hipEvent_t ev[2];
+      for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
+      best.ms = INFINITY;
+      for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE}) {
+        for (int32_t b : {64, 128, 256}) {
+          e.configure(p, b, st);
+          e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);  // code objects loaded
+          hip_check(hipEventRecord(ev[0], st), "record");
+          for (int32_t r = 0; r < reps; ++r) e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);
+          hip_check(hipEventRecord(ev[1], st), "record");
+          hip_check(hipEventSynchronize(ev[1]), "event sync");
+          float t = 0.f;
+          hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "elapsed");
+          const double ms = (double)t / reps;
+          if (ms < best.ms) best = TuneChoice{p, b, ms};
+        }
+      }
+      for (auto& x : ev) (void)hipEventDestroy(x);
+      launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tune[key] = best;
+    }
+    e.configure(best.path, best.block, st);
+    // the captured chunk holds the old launch shapes
+    if (s->chunk) {
+      (void)hipGraphExecDestroy(s->chunk);
+      s->chunk = nullptr;
+    }
+    if (path) *path = best.path;
+    if (block) *block = best.block;
+    if (us_per_stage) *us_per_stage = best.ms * 1e3;
+  });
+}
+
+esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, void* stream) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    if (path != ESR_STAGE_CELL && path != ESR_STAGE_FACE)
+      throw Error(ESR_ERR_VALUE, "stage path must be ESR_STAGE_CELL or ESR_STAGE_FACE");
+    if (block != 64 && block != 128 && block != 256) throw Error(ESR_ERR_VALUE, "block must be 64, 128 or 256");
+    if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
+    s->eng.configure(path, block, S(stream));
+    if (s->chunk) {
+      (void)hipGraphExecDestroy(s->chunk);
+      s->chunk = nullptr;
+    }
   });
 }
 

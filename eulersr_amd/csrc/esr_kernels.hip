@@ -104,7 +104,7 @@ __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, co
                                          const CaseDev* __restrict__ cases, Item& it) {
   const GroupDev* G = groups + blk_group[blockIdx.x];
   const int32_t C = G->C;
-  const int64_t t = (int64_t)(blockIdx.x - (FACES ? G->fblk0 : G->blk0)) * BLOCK + threadIdx.x;
+  const int64_t t = (int64_t)(blockIdx.x - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
   const int32_t idx = (int32_t)(t / C);
   const int32_t c = (int32_t)(t - (int64_t)idx * C);
   if (idx >= (FACES ? G->m.nf : G->m.n)) return false;
@@ -1085,7 +1085,7 @@ static void check_launch(const char* what) { hip_check(hipGetLastError(), what);
 
 void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st) {
   if (ls.nblocks <= 0) return;
-  hipLaunchKernelGGL(k_recon, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, Ws, rec);
+  hipLaunchKernelGGL(k_recon, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, Ws, rec);
   check_launch("k_recon");
 }
 
@@ -1093,10 +1093,10 @@ void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws,
   if (ls.fnblocks <= 0) return;
   CaseDev* cm = const_cast<CaseDev*>(cases);
   if (ls.recon == ESR_RECON_MUSCL)
-    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.fnblocks), dim3(BLOCK), 0, st, ls.groups, ls.fblk_group,
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.fnblocks), dim3(ls.block), 0, st, ls.groups, ls.fblk_group,
                        cm, Ws, rec);
   else
-    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.fnblocks), dim3(BLOCK), 0, st, ls.groups,
+    hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.fnblocks), dim3(ls.block), 0, st, ls.groups,
                        ls.fblk_group, cm, Ws, rec);
   check_launch("k_lf_cmax");
 }
@@ -1104,7 +1104,7 @@ void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws,
 template <int RECON, int ENT>
 static void launch_face_f(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, float4* fx,
                           hipStream_t st) {
-  const dim3 g(ls.fnblocks), b(BLOCK);
+  const dim3 g(ls.fnblocks), b(ls.block);
   switch (ls.flux) {
     case ESR_FLUX_LF:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_LF, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec, fx);
@@ -1153,7 +1153,7 @@ void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, co
 template <int RECON, int ENT>
 static void launch_cell_f(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
                           const float4* rec, float4* out, float4* acc, hipStream_t st) {
-  const dim3 g(ls.nblocks), b(BLOCK);
+  const dim3 g(ls.nblocks), b(ls.block);
 #define ESR_CELL_CASE(FL)                                                                                    \
   case FL:                                                                                                   \
     hipLaunchKernelGGL((k_cell<RECON, FL, ENT>), g, b, 0, st, ls.groups, ls.blk_group, cases, Ws, W0, rec, out, \
@@ -1194,7 +1194,7 @@ void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const fl
   if (ls.nblocks <= 0) return;
 #define ESR_UPD_CASE(MD)                                                                                   \
   case MD:                                                                                                 \
-    hipLaunchKernelGGL((k_update<MD>), dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, \
+    hipLaunchKernelGGL((k_update<MD>), dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, \
                        Ws, W0, fx, out, acc);                                                              \
     break;
   switch (mode) {
@@ -1295,7 +1295,7 @@ void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, c
 
 void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src, float4* dst, hipStream_t st) {
   if (ls.nblocks <= 0) return;
-  hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(BLOCK), 0, st, ls.groups, ls.blk_group, cases, src, dst);
+  hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, src, dst);
   check_launch("k_commit");
 }
 

@@ -16,9 +16,10 @@ struct LaunchSet {
   int32_t flux = ESR_FLUX_HLLC;
   int32_t recon = ESR_RECON_MUSCL;
   int32_t entropy = 0;
+  int32_t block = 256;  // threads per block of the cell and face kernels (64, 128 or 256)
 };
 
-constexpr int BLOCK = 256;
+constexpr int BLOCK = 256;  // maximum block of the stage kernels; block of the reductions
 
 void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st);
 void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec,

@@ -49,8 +49,10 @@ def get_stage_path() -> str:
 
 class BatchSolver:
     def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
-                 graph=True):
+                 graph=True, autotune=False):
         require_gpu()
+        self.autotune = bool(autotune)
+        self.tuning = None
         self.specs = list(cases)
         n = len(self.specs)
         arr = (L.Case * n)()
@@ -104,6 +106,23 @@ class BatchSolver:
 
     def prepare(self):
         L.check(L.lib().esr_solver_prepare(self.handle, stream_ptr()), "esr_solver_prepare")
+        if self.autotune:
+            self.tune()
+
+    def set_launch(self, path="cell", block=256):
+        """Pins the stage path ("cell" / "face") and block size (64/128/256)."""
+        code = {"cell": L.STAGE_CELL, "face": L.STAGE_FACE}[path]
+        L.check(L.lib().esr_solver_set_launch(self.handle, code, int(block), stream_ptr()), "esr_solver_set_launch")
+
+    def tune(self, reps=10):
+        """Autotuner: fastest stage path and block size for this solver's
+        launch shape (cached per shape by the library)."""
+        p, b, us = C.c_int32(), C.c_int32(), C.c_double()
+        L.check(L.lib().esr_solver_autotune(self.handle, int(reps), C.byref(p), C.byref(b), C.byref(us),
+                                            stream_ptr()), "esr_solver_autotune")
+        self.tuning = dict(path={L.STAGE_CELL: "cell", L.STAGE_FACE: "face"}[p.value], block=b.value,
+                           us_per_stage=us.value)
+        return self.tuning
 
     def run(self, until_step=0):
         s = C.c_int64()

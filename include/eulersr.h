@@ -262,6 +262,20 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
 enum esr_stage_path { ESR_STAGE_AUTO = 0, ESR_STAGE_CELL = 1, ESR_STAGE_FACE = 2 };
 esr_status esr_set_stage_path(int32_t path);
 int32_t esr_get_stage_path(void);
+
+/* Autotuner: times one SSP-RK2 stage of the prepared solver on its current
+ * states for each stage path (cell, face) x block size (64, 128, 256 threads)
+ * with HIP events, `reps` launches each, and keeps the fastest; the choice
+ * is cached per launch shape (kernel configurations and work-item counts) for
+ * the process, so later solvers of the same shape reuse it without timing.
+ * Outputs (any may be null): the chosen path and block, and the device time
+ * of one stage in microseconds. No counterpart in the reference. */
+esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, double* us_per_stage,
+                               void* stream);
+/* Pins the launch configuration of a solver: stage path (ESR_STAGE_CELL or
+ * ESR_STAGE_FACE) and block size (64, 128 or 256). Results do not depend on
+ * the block size. No counterpart in the reference. */
+esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, void* stream);
 const char* esr_last_error(void);
 int32_t esr_version(void);
 
