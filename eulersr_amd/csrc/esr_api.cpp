@@ -128,6 +128,7 @@ struct Engine {
   DevBuf W0, W1, W2, acc, Wref, rec, fx;
   int32_t rstride = 1;
   DevBuf part_d, part_r, part_f, nranges, cfl_d, dt_d, forces_d, probe_d, count_d;
+  DevBuf counter_d;  // uint32 blocks-done counter of the fused check (zero between launches)
   std::vector<std::unique_ptr<DevBuf>> custom_sponge;
   int32_t* count_h = nullptr;  // pinned
 
@@ -300,6 +301,8 @@ struct Engine {
     forces_d.alloc(2 * (size_t)n_cases * sizeof(double));
     probe_d.alloc(n_cases * sizeof(double));
     count_d.alloc(sizeof(int32_t));
+    counter_d.alloc(sizeof(uint32_t));
+    hip_check(hipMemset(counter_d.p, 0, sizeof(uint32_t)), "counter memset");
     hip_check(hipHostMalloc((void**)&count_h, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
     *count_h = 0;
   }
@@ -364,10 +367,11 @@ struct Engine {
   }
 
   // norms of (Wn - Wref) and Wref for cases flagged 'check'
-  void norms(const float4* Wn, float4* Wref_, int copy_ref, hipStream_t st) {
+  // fused: the check arguments, run by the last block of the last launch
+  void norms(const float4* Wn, float4* Wref_, int copy_ref, hipStream_t st, const CheckArgs* fused = nullptr) {
     for (size_t gi = 0; gi < groups.size(); ++gi)
       launch_norms(groups[gi], gdev((int32_t)gi), cases(), Wn, Wref_, part_d.as<double>(), part_r.as<double>(),
-                   rstride, copy_ref, st);
+                   rstride, copy_ref, st, gi + 1 == groups.size() ? fused : nullptr);
   }
   void forces(const float4* W, hipStream_t st) {
     for (size_t gi = 0; gi < groups.size(); ++gi)
@@ -477,6 +481,7 @@ struct esr_solver {
   hipStream_t gs = nullptr;
   hipGraphExec_t chunk = nullptr;
   DevBuf dstep;  // int64[2]: step before / after the chunk
+  int64_t dstep_at = -1;  // value of dstep[0] on the device (-1: unknown)
   ~esr_solver() {
     if (chunk) (void)hipGraphExecDestroy(chunk);
     if (gs) (void)hipStreamDestroy(gs);
@@ -746,6 +751,7 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream) {
     launch_copy(e.W0.as<float4>(), e.Wref.as<float4>(), e.elems, st);
     s->step = 0;
     s->prepared = true;
+    s->dstep_at = -1;
     s->finalized = false;
   });
 }
@@ -755,14 +761,32 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream) {
 namespace {
 
 // one stationarity check after step k (host step, or the device counter)
+// (forces first, then the norms, then the check)
+constexpr int64_t kFusedCheckPartials = 4096;
 void enqueue_check(esr_solver* s, int64_t k, bool eng_on, hipStream_t st, int64_t* dstep) {
   Engine& e = s->eng;
   const int32_t K = s->so.check_every;
   launch_mark_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), k, K, 0, st, dstep);
-  e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st);
   if (eng_on) e.forces(e.W0.as<float4>(), st);
-  launch_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), e.part_d.as<double>(), e.part_r.as<double>(),
-               e.nranges.as<int32_t>(), e.rstride, eng_on ? e.forces_d.as<double>() : nullptr, k, K, st, dstep);
+  const double* forces = eng_on ? e.forces_d.as<double>() : nullptr;
+  // small batches: the last block of the norms runs the checks (one launch
+  // less); large ones: a wave per case in a launch of its own
+  if ((int64_t)e.n_cases * e.rstride <= kFusedCheckPartials) {
+    CheckArgs ca{};
+    ca.n_cases = e.n_cases;
+    ca.cases = e.cases();
+    ca.loops = e.loops_d.as<CaseLoop>();
+    ca.nranges = e.nranges.as<int32_t>();
+    ca.forces = forces;
+    ca.step = k;
+    ca.dstep = dstep;
+    ca.counter = e.counter_d.as<uint32_t>();
+    e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st, &ca);
+  } else {
+    e.norms(e.W0.as<float4>(), e.Wref.as<float4>(), 1, st);
+    launch_check(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), e.part_d.as<double>(), e.part_r.as<double>(),
+                 e.nranges.as<int32_t>(), e.rstride, forces, k, K, st, dstep);
+  }
 }
 
 void capture_chunk(esr_solver* s, bool eng_on) {
@@ -818,9 +842,11 @@ extern "C" esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t*
       bool checked = false;
       if (use_graph && k0 % K == 0 && k0 + K <= limit && !n_inside) {
         if (!s->chunk) capture_chunk(s, eng_on);
-        launch_set_step(s->dstep.as<int64_t>(), k0, st);
+        // the device step counter already holds k0 after a chunk
+        if (s->dstep_at != k0) launch_set_step(s->dstep.as<int64_t>(), k0, st);
         hip_check(hipGraphLaunch(s->chunk, st), "graph launch");
         s->step = k0 + K;
+        s->dstep_at = s->step;
         checked = true;
       } else {
         s->step = k0 + 1;
@@ -903,7 +929,7 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
     out->engineering_converged = L.eng_stop;
     out->engineering_stopping_step = L.eng_stopping_step;
     out->diverged = L.diverged;
-    const int32_t nr = std::min(std::min(L.patience, L.n_checks), 4);
+    const int32_t nr = std::min(L.n_checks, RECENT_CHECKS);
     double rm = nr > 0 ? 0.0 : NAN;
     for (int32_t q = 0; q < nr; ++q) rm = std::max(rm, L.recent[q]);
     out->recent_rel_max = rm;

@@ -142,8 +142,9 @@ struct CaseLoop {
   int32_t eng_stop;
   int32_t diverged;
   int32_t n_checks;
-  double recent[4];  // the last four check residuals, newest first
+  double recent[16];  // check residuals, ring buffer indexed by n_checks % 16
 };
+constexpr int32_t RECENT_CHECKS = 16;
 
 enum StageMode : int {
   MODE_RESID = 0,   // out = R

@@ -790,10 +790,17 @@ __device__ __forceinline__ RedMap red_map(const GroupDev* G) {
   return r;
 }
 
+__device__ void check_cases(int32_t c0, int32_t c1, CaseDev* cases, CaseLoop* loops, const double* part_d,
+                            const double* part_r, const int32_t* __restrict__ nranges, int32_t rstride,
+                            const double* __restrict__ forces, int64_t step);
+
+// ca.counter != null: the last block of this launch to finish (every earlier
+// launch of the stream has completed) runs the stationarity check of every
+// case, then resets the counter -- one launch less per check.
 __global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ gdev, const CaseDev* __restrict__ cases,
                                                  const float4* __restrict__ Wn, float4* __restrict__ Wref,
                                                  double* __restrict__ part_d, double* __restrict__ part_r,
-                                                 int32_t rstride, int copy_ref) {
+                                                 int32_t rstride, int copy_ref, CheckArgs ca) {
   const GroupDev* G = gdev;
   const RedMap r = red_map(G);
   __shared__ double sd[BLOCK], sr[BLOCK];
@@ -828,6 +835,22 @@ __global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ gd
     }
     part_d[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = a;
     part_r[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = b;
+  }
+  if (!ca.counter) return;
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = (atomicAdd(ca.counter, 1u) == gridDim.x * gridDim.y - 1) ? 1 : 0;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // agent-scope acquire: the other blocks' partials are visible
+  int64_t step = ca.step;
+  if (ca.dstep) step = ca.dstep[1];
+  check_cases(0, ca.n_cases, ca.cases, ca.loops, part_d, part_r, ca.nranges, rstride, ca.forces, step);
+  __syncthreads();
+  if (t == 0) {
+    if (ca.dstep) ca.dstep[0] = step;
+    *ca.counter = 0u;
   }
 }
 
@@ -907,24 +930,33 @@ __global__ __launch_bounds__(BLOCK) void k_forces(const GroupDev* __restrict__ g
   }
 }
 
-// Stationarity check of main.py:149-177 for every case whose check flag is set.
-__global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* __restrict__ part_d,
-                        const double* __restrict__ part_r, const int32_t* __restrict__ nranges, int32_t rstride,
-                        const double* __restrict__ forces, int64_t step, int32_t K, int64_t* dstep) {
-  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (dstep) {
-    step = dstep[1];
-    if (c == 0) dstep[0] = step;
+// Sums of case c's partial norms by one wave: lane l adds ranges l, l+64,
+// ... in order, then a fixed shuffle tree; every lane returns the totals.
+__device__ __forceinline__ void wave_case_sums(int32_t c, const double* part_d, const double* part_r,
+                                               const int32_t* __restrict__ nranges, int32_t rstride, double& d,
+                                               double& r) {
+  const int lane = threadIdx.x & 63;
+  double a = 0.0, b = 0.0;
+  const int32_t nr = nranges[c];
+  for (int32_t q = lane; q < nr; q += 64) {
+    a += part_d[(int64_t)c * rstride + q];
+    b += part_r[(int64_t)c * rstride + q];
   }
-  if (c >= n_cases) return;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  d = __shfl(a, 0, 64);
+  r = __shfl(b, 0, 64);
+}
+
+// Stationarity check of main.py:149-177 for case c (check flag set), given
+// the FP64 sums d = ||W - W_ref||^2, r = ||W_ref||^2.
+__device__ void apply_check(int32_t c, CaseDev* cases, CaseLoop* loops, double d, double r,
+                            const double* __restrict__ forces, int64_t step) {
   CaseDev& cs = cases[c];
   CaseLoop& L = loops[c];
-  if (!cs.check) return;
-  double d = 0.0, r = 0.0;
-  for (int32_t q = 0; q < nranges[c]; ++q) {
-    d += part_d[(int64_t)c * rstride + q];
-    r += part_r[(int64_t)c * rstride + q];
-  }
   const int64_t block_steps = (step - L.ref_step) > 1 ? (step - L.ref_step) : 1;
   const double rel = sqrt(d) / ((double)block_steps * (sqrt(r) + 1e-16));
   const bool under = rel <= L.threshold;
@@ -937,10 +969,7 @@ __global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const 
   L.streak_start = new_start;
   L.ref_step = step;
   L.last_rel = rel;
-  L.recent[3] = L.recent[2];
-  L.recent[2] = L.recent[1];
-  L.recent[1] = L.recent[0];
-  L.recent[0] = rel;
+  L.recent[L.n_checks % RECENT_CHECKS] = rel;
   L.n_checks += 1;
   if (!(d == d) || isinf(d)) L.diverged = 1;
   if (L.track_eng && forces) {
@@ -963,6 +992,32 @@ __global__ void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const 
     cs.active = 0;
   }
   cs.check = 0;
+}
+
+// checks of cases [c0, c1) with the waves of this block (wave w: cases
+// c0 + w, c0 + w + waves, ...)
+__device__ void check_cases(int32_t c0, int32_t c1, CaseDev* cases, CaseLoop* loops, const double* part_d,
+                            const double* part_r, const int32_t* __restrict__ nranges, int32_t rstride,
+                            const double* __restrict__ forces, int64_t step) {
+  const int wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+  for (int32_t c = c0 + wave; c < c1; c += waves) {
+    if (!cases[c].check) continue;  // wave-uniform
+    double d, r;
+    wave_case_sums(c, part_d, part_r, nranges, rstride, d, r);
+    if ((threadIdx.x & 63) == 0) apply_check(c, cases, loops, d, r, forces, step);
+  }
+}
+
+// one wave per case, four cases per block
+__global__ __launch_bounds__(256) void k_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops,
+                                               const double* __restrict__ part_d, const double* __restrict__ part_r,
+                                               const int32_t* __restrict__ nranges, int32_t rstride,
+                                               const double* __restrict__ forces, int64_t step, int32_t K,
+                                               int64_t* dstep) {
+  if (dstep) step = dstep[1];
+  const int32_t c0 = blockIdx.x * 4;
+  check_cases(c0, min(c0 + 4, n_cases), cases, loops, part_d, part_r, nranges, rstride, forces, step);
+  if (dstep && blockIdx.x == 0 && threadIdx.x == 0) dstep[0] = step;
 }
 
 __global__ void k_dt_final(int32_t n_cases, CaseDev* cases, const float* __restrict__ part,
@@ -1219,9 +1274,12 @@ static dim3 red_grid(const GroupDev& g) {
 }
 
 void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn, float4* Wref,
-                  double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st) {
+                  double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st,
+                  const CheckArgs* fused) {
+  CheckArgs ca{};
+  if (fused) ca = *fused;
   hipLaunchKernelGGL(k_norms, red_grid(g), dim3(BLOCK), 0, st, gdev, cases, Wn, Wref, part_d, part_r, rstride,
-                     copy_ref);
+                     copy_ref, ca);
   check_launch("k_norms");
 }
 
@@ -1240,7 +1298,7 @@ void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases
 void launch_check(int32_t n_cases, CaseDev* cases, CaseLoop* loops, const double* part_d, const double* part_r,
                   const int32_t* nranges, int32_t rstride, const double* forces, int64_t step, int32_t K,
                   hipStream_t st, int64_t* dstep) {
-  hipLaunchKernelGGL(k_check, dim3((n_cases + 127) / 128), dim3(128), 0, st, n_cases, cases, loops, part_d, part_r,
+  hipLaunchKernelGGL(k_check, dim3((n_cases + 3) / 4), dim3(256), 0, st, n_cases, cases, loops, part_d, part_r,
                      nranges, rstride, forces, step, K, dstep);
   check_launch("k_check");
 }

@@ -36,8 +36,20 @@ void launch_cell(const LaunchSet& ls, int mode, const CaseDev* cases, const floa
 // Per-case reductions over one group (host-side group descriptor `g`,
 // device copy at `gdev`). Partial results at part[case * rstride + r].
 int32_t reduce_ranges(const GroupDev& g);
+// Arguments of a stationarity check fused into the last norms launch.
+struct CheckArgs {
+  int32_t n_cases;
+  CaseDev* cases;
+  CaseLoop* loops;
+  const int32_t* nranges;
+  const double* forces;
+  int64_t step;
+  int64_t* dstep;
+  uint32_t* counter;  // zero between launches; null: no fused check
+};
 void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn,
-                  float4* Wref, double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st);
+                  float4* Wref, double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st,
+                  const CheckArgs* fused = nullptr);
 void launch_dt(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, float* part,
                int32_t rstride, int local, hipStream_t st);
 void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W,

@@ -107,10 +107,11 @@ def with_targets(specs, ref_reports, tf=1.0e4):
     N, ends the solve, and threshold = 1.001 x the reference field's residual
     (eval/convergence_case.py:61-66). The reference takes that residual from a
     one-step probe and expects the reference field to pass its own check at
-    once; with the limiter's residual plateau a one-step probe can fall below
-    the 5-step check residual of the same field, so the residual used here is
-    the larger of the probe and the reference run's last `patience` check
-    residuals."""
+    once; with the limiter's residual plateau (a limit cycle whose check
+    residual wanders within a band) a one-step probe or a single check can
+    sit at the bottom of that band, so the residual used here is the larger of
+    the probe and the reference run's largest residual over its last 16
+    checks (80 steps)."""
     out = []
     for sp, r in zip(specs, ref_reports):
         s = copy.copy(sp)

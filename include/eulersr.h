@@ -168,7 +168,7 @@ typedef struct {
   int32_t engineering_converged;
   int64_t engineering_stopping_step;
   int32_t diverged;          /* non-finite residual at a check: the case stops there, unconverged */
-  double recent_rel_max;     /* largest stationarity residual of the last min(patience, 4) checks */
+  double recent_rel_max;     /* largest stationarity residual of the last min(16, checks) checks */
 } esr_case_report;
 
 esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_solve_opts* so,
