@@ -85,6 +85,25 @@ PATH_KERNELS = {"cell": ("k_recon", "k_cell<MUSCL,HLLC,stage1>"),
                 "face": ("k_recon", "k_face<MUSCL,HLLC>", "k_update<stage1>")}
 
 
+# rocprofv3 names of the timed kernels (template arguments: recon 1 = MUSCL,
+# flux 4 = HLLC, entropy 0; update mode 2 = stage 1)
+PROFILE_NAMES = {"k_recon": "k_recon", "k_face<MUSCL,HLLC>": "k_face<1, 4, 0>", "k_update<stage1>": "k_update<2>",
+                 "k_cell<MUSCL,HLLC,stage1>": "k_cell<1, 4, 0>"}
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_bench_6case_h0.025.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` at the bench size from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
+    copy), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    from eulersr_amd.profiling import read_traffic
+    t = read_traffic(TRAFFIC_FILE).get(PROFILE_NAMES.get(kernel, kernel))
+    return None if t is None else t[0] + t[1]
+
+
 def kernel_table(sizes, ms):
     kern = {}
     for (name, fn), t in zip(KERNELS, ms):
@@ -366,7 +385,7 @@ def main():
                                "and a 4x256 MLP trained on 55 solver-generated LR/HR pairs per geometry",
                    "cases": len(cases), "cells": cells, "h": args.h, "h_lr": args.h_lr},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                      "kernel": dom, "stage_path": path, "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,

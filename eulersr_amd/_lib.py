@@ -103,9 +103,9 @@ SIGNATURES = {
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
     "esr_set_stage_path": (C.c_int, [C.c_int32]),
-    "esr_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]),
+    "esr_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),
     "esr_solver_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
-                                      C.POINTER(C.c_double), C.c_void_p]),
+                                      C.POINTER(C.c_int32), C.POINTER(C.c_double), C.c_void_p]),
     "esr_get_stage_path": (C.c_int32, []),
     "esr_mlp_create": (C.c_int, [C.c_int32, i32p, C.c_int64, C.POINTER(C.c_void_p)]),
     "esr_mlp_destroy": (C.c_int, [C.c_void_p]),

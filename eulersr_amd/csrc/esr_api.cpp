@@ -94,7 +94,7 @@ struct CaseSpec {
 // Device state of a set of cases.
 // autotuned launch choices by launch shape (esr_solver_autotune)
 struct TuneChoice {
-  int32_t path = ESR_STAGE_CELL, block = 256;
+  int32_t path = ESR_STAGE_CELL, block = 256, xcd = 1;
   double ms = 0.0;
 };
 std::mutex g_tune_mu;
@@ -138,7 +138,7 @@ struct Engine {
 
   // cell / face block tables of a class for `block` threads per block
   // (host group copies updated; the caller uploads groups_d)
-  void block_tables(Cls& c, int32_t block) {
+  void block_tables(Cls& c, int32_t block, int32_t xcd = 1) {
     std::vector<int32_t> blk, fblk;
     for (int32_t gi : c.gids) {
       GroupDev& g = groups[gi];
@@ -151,19 +151,24 @@ struct Engine {
       g.fnblk = (int32_t)((fitems + block - 1) / block);
       for (int32_t b = 0; b < g.fnblk; ++b) fblk.push_back(gi);
     }
+    // launches cover a multiple of 8 blocks (one run per XCD); padding -1
+    while (blk.size() % 8) blk.push_back(-1);
+    while (fblk.size() % 8) fblk.push_back(-1);
+    const int32_t nb = (int32_t)blk.size(), nfb = (int32_t)fblk.size();
     upload(c.blk, blk);
     upload(c.fblk, fblk);
     c.ls.blk_group = c.blk.as<int32_t>();
-    c.ls.nblocks = (int32_t)blk.size();
+    c.ls.nblocks = nb;
     c.ls.fblk_group = c.fblk.as<int32_t>();
-    c.ls.fnblocks = (int32_t)fblk.size();
+    c.ls.fnblocks = nfb;
     c.ls.block = block;
+    c.ls.xcd = xcd ? 1 : 0;
   }
 
   // one launch configuration for every class: stage path and block size
-  void configure(int32_t path, int32_t block, hipStream_t st) {
+  void configure(int32_t path, int32_t block, int32_t xcd, hipStream_t st) {
     hip_check(hipStreamSynchronize(st), "configure sync");
-    for (auto& c : classes) block_tables(*c, block);
+    for (auto& c : classes) block_tables(*c, block, xcd);
     hip_check(hipMemcpy(groups_d.p, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice),
               "groups H2D");
     stage_path = path;
@@ -954,8 +959,8 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
   });
 }
 
-esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, double* us_per_stage,
-                               void* stream) {
+esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, int32_t* xcd_order,
+                               double* us_per_stage, void* stream) {
   return guard([&] {
     if (!s || reps < 1) throw Error(ESR_ERR_VALUE, "bad arguments");
     if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
@@ -989,7 +994,8 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
       best.ms = INFINITY;
       for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE}) {
         for (int32_t b : {64, 128, 256}) {
-          e.configure(p, b, st);
+         for (int32_t x : {1, 0}) {
+          e.configure(p, b, x, st);
           e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);  // code objects loaded
           hip_check(hipEventRecord(ev[0], st), "record");
           for (int32_t r = 0; r < reps; ++r) e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);
@@ -998,7 +1004,8 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
           float t = 0.f;
           hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "elapsed");
           const double ms = (double)t / reps;
-          if (ms < best.ms) best = TuneChoice{p, b, ms};
+          if (ms < best.ms) best = TuneChoice{p, b, x, ms};
+         }
         }
       }
       for (auto& x : ev) (void)hipEventDestroy(x);
@@ -1006,7 +1013,7 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
       std::lock_guard<std::mutex> lk(g_tune_mu);
       g_tune[key] = best;
     }
-    e.configure(best.path, best.block, st);
+    e.configure(best.path, best.block, best.xcd, st);
     // the captured chunk holds the old launch shapes
     if (s->chunk) {
       (void)hipGraphExecDestroy(s->chunk);
@@ -1014,18 +1021,19 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
     }
     if (path) *path = best.path;
     if (block) *block = best.block;
+    if (xcd_order) *xcd_order = best.xcd;
     if (us_per_stage) *us_per_stage = best.ms * 1e3;
   });
 }
 
-esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, void* stream) {
+esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, int32_t xcd_order, void* stream) {
   return guard([&] {
     if (!s) throw Error(ESR_ERR_VALUE, "null solver");
     if (path != ESR_STAGE_CELL && path != ESR_STAGE_FACE)
       throw Error(ESR_ERR_VALUE, "stage path must be ESR_STAGE_CELL or ESR_STAGE_FACE");
     if (block != 64 && block != 128 && block != 256) throw Error(ESR_ERR_VALUE, "block must be 64, 128 or 256");
     if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
-    s->eng.configure(path, block, S(stream));
+    s->eng.configure(path, block, xcd_order ? 1 : 0, S(stream));
     if (s->chunk) {
       (void)hipGraphExecDestroy(s->chunk);
       s->chunk = nullptr;

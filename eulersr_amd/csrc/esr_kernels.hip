@@ -99,12 +99,28 @@ struct Item {
   int64_t sidx; // state index (cell kernels)
 };
 
+// XCD-aware block order (a tuning choice): the hardware deals workgroup p to
+// XCD p % 8, so the logical block (p % 8) * (grid / 8) + p / 8 gives every
+// XCD (and its L2) a contiguous run of logical blocks -- in RCM order, a
+// contiguous run of cells whose neighbours are mostly in the same run. Block
+// tables hold a multiple of 8 entries (padding: group -1); `xcd` (a kernel
+// argument) selects the order.
+constexpr int32_t N_XCD = 8;
+__device__ __forceinline__ int32_t logical_block(int xcd) {
+  if (!xcd) return (int32_t)blockIdx.x;
+  const int32_t per = gridDim.x / N_XCD;
+  return (int32_t)(blockIdx.x % N_XCD) * per + (int32_t)(blockIdx.x / N_XCD);
+}
+
 template <bool FACES>
 __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const int32_t* __restrict__ blk_group,
-                                         const CaseDev* __restrict__ cases, Item& it) {
-  const GroupDev* G = groups + blk_group[blockIdx.x];
+                                         const CaseDev* __restrict__ cases, int xcd, Item& it) {
+  const int32_t lb = logical_block(xcd);
+  const int32_t gi = blk_group[lb];
+  if (gi < 0) return false;
+  const GroupDev* G = groups + gi;
   const int32_t C = G->C;
-  const int64_t t = (int64_t)(blockIdx.x - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)(lb - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
   const int32_t idx = (int32_t)(t / C);
   const int32_t c = (int32_t)(t - (int64_t)idx * C);
   if (idx >= (FACES ? G->m.nf : G->m.n)) return false;
@@ -138,9 +154,9 @@ __device__ __forceinline__ float venkat(float a, float b) {
 __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ groups,
                                                  const int32_t* __restrict__ blk_group,
                                                  const CaseDev* __restrict__ cases,
-                                                 const float4* __restrict__ Ws, float4* __restrict__ rec) {
+                                                 const float4* __restrict__ Ws, float4* __restrict__ rec, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
@@ -585,9 +601,9 @@ template <int RECON, int FLUX, int ENT>
 __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ groups,
                                                 const int32_t* __restrict__ blk_group,
                                                 const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
-                                                const float4* __restrict__ rec, float4* __restrict__ fx) {
+                                                const float4* __restrict__ rec, float4* __restrict__ fx, int xcd) {
   Item it;
-  if (!map_item<true>(groups, blk_group, cases, it)) return;
+  if (!map_item<true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const CaseDev& cs = cases[G->case0 + it.c];
@@ -606,9 +622,9 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
 template <int RECON>
 __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ groups,
                                                    const int32_t* __restrict__ blk_group, CaseDev* cases,
-                                                   const float4* __restrict__ Ws, const float4* __restrict__ rec) {
+                                                   const float4* __restrict__ Ws, const float4* __restrict__ rec, int xcd) {
   Item it;
-  if (!map_item<true>(groups, blk_group, cases, it)) return;
+  if (!map_item<true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const Phys& ph = G->ph;
   CaseDev& cs = cases[G->case0 + it.c];
@@ -684,9 +700,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
                                                   const int32_t* __restrict__ blk_group,
                                                   const CaseDev* __restrict__ cases,
                                                   const float4* __restrict__ Ws, const float4* W0,
-                                                  const float4* __restrict__ fx, float4* out, float4* acc) {
+                                                  const float4* __restrict__ fx, float4* out, float4* acc, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const int32_t n = m.n, cell = it.idx;
@@ -713,9 +729,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
                                                 const int32_t* __restrict__ blk_group,
                                                 const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
                                                 const float4* W0, const float4* __restrict__ rec, float4* out,
-                                                float4* acc, int mode) {
+                                                float4* acc, int mode, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
@@ -1126,9 +1142,9 @@ __global__ void k_probe_final(int32_t n_cases, CaseDev* cases, const double* __r
 __global__ __launch_bounds__(BLOCK) void k_commit(const GroupDev* __restrict__ groups,
                                                   const int32_t* __restrict__ blk_group,
                                                   const CaseDev* __restrict__ cases, const float4* __restrict__ src,
-                                                  float4* __restrict__ dst) {
+                                                  float4* __restrict__ dst, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, it)) return;
+  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
   dst[it.sidx] = src[it.sidx];
 }
 
@@ -1140,7 +1156,8 @@ static void check_launch(const char* what) { hip_check(hipGetLastError(), what);
 
 void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st) {
   if (ls.nblocks <= 0) return;
-  hipLaunchKernelGGL(k_recon, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, Ws, rec);
+  hipLaunchKernelGGL(k_recon, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, Ws, rec,
+                     ls.xcd);
   check_launch("k_recon");
 }
 
@@ -1149,10 +1166,10 @@ void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws,
   CaseDev* cm = const_cast<CaseDev*>(cases);
   if (ls.recon == ESR_RECON_MUSCL)
     hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_MUSCL>, dim3(ls.fnblocks), dim3(ls.block), 0, st, ls.groups, ls.fblk_group,
-                       cm, Ws, rec);
+                       cm, Ws, rec, ls.xcd);
   else
     hipLaunchKernelGGL(k_lf_cmax<ESR_RECON_CONSTANT>, dim3(ls.fnblocks), dim3(ls.block), 0, st, ls.groups,
-                       ls.fblk_group, cm, Ws, rec);
+                       ls.fblk_group, cm, Ws, rec, ls.xcd);
   check_launch("k_lf_cmax");
 }
 
@@ -1162,27 +1179,27 @@ static void launch_face_f(const LaunchSet& ls, const CaseDev* cases, const float
   const dim3 g(ls.fnblocks), b(ls.block);
   switch (ls.flux) {
     case ESR_FLUX_LF:
-      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_LF, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec, fx);
+      hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_LF, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec, fx, ls.xcd);
       break;
     case ESR_FLUX_RUSANOV:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_RUSANOV, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws,
-                         rec, fx);
+                         rec, fx, ls.xcd);
       break;
     case ESR_FLUX_TADMOR:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_TADMOR, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws,
-                         rec, fx);
+                         rec, fx, ls.xcd);
       break;
     case ESR_FLUX_AUSM:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_AUSM, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
-                         fx);
+                         fx, ls.xcd);
       break;
     case ESR_FLUX_HLLC:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_HLLC, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
-                         fx);
+                         fx, ls.xcd);
       break;
     case ESR_FLUX_ROE:
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_ROE, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
-                         fx);
+                         fx, ls.xcd);
       break;
     default: throw Error(ESR_ERR_VALUE, "unknown flux");
   }
@@ -1212,7 +1229,7 @@ static void launch_cell_f(const LaunchSet& ls, int mode, const CaseDev* cases, c
 #define ESR_CELL_CASE(FL)                                                                                    \
   case FL:                                                                                                   \
     hipLaunchKernelGGL((k_cell<RECON, FL, ENT>), g, b, 0, st, ls.groups, ls.blk_group, cases, Ws, W0, rec, out, \
-                       acc, mode);                                                                           \
+                       acc, mode, ls.xcd);                                                                   \
     break;
   switch (ls.flux) {
     ESR_CELL_CASE(ESR_FLUX_LF)
@@ -1250,7 +1267,7 @@ void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const fl
 #define ESR_UPD_CASE(MD)                                                                                   \
   case MD:                                                                                                 \
     hipLaunchKernelGGL((k_update<MD>), dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, \
-                       Ws, W0, fx, out, acc);                                                              \
+                       Ws, W0, fx, out, acc, ls.xcd);                                                              \
     break;
   switch (mode) {
     ESR_UPD_CASE(MODE_RESID)
@@ -1353,7 +1370,8 @@ void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, c
 
 void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src, float4* dst, hipStream_t st) {
   if (ls.nblocks <= 0) return;
-  hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, src, dst);
+  hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, src, dst,
+                     ls.xcd);
   check_launch("k_commit");
 }
 

@@ -17,6 +17,7 @@ struct LaunchSet {
   int32_t recon = ESR_RECON_MUSCL;
   int32_t entropy = 0;
   int32_t block = 256;  // threads per block of the cell and face kernels (64, 128 or 256)
+  int32_t xcd = 1;      // XCD-contiguous block order (the flag after each block table)
 };
 
 constexpr int BLOCK = 256;  // maximum block of the stage kernels; block of the reductions

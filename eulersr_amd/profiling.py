@@ -41,24 +41,34 @@ def trace_summary(path, tail=50):
     return "\n".join(out)
 
 
-def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_rocclr_copyBuffer"):
-    """Per-kernel FETCH_SIZE / WRITE_SIZE means (KiB per dispatch). With
+def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_rocclr_copyBuffer", tail=50,
+                traffic_out=None):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch): mean over all
+    dispatches and over the last `tail` dispatches (bench.py ends with
+    back-to-back launches of the stage kernels at the bench size). With
     copy_bytes, the largest dispatch of `calib_kernel` (a device copy of
-    copy_bytes bytes each way) gives the factor from counter KiB to bytes."""
+    copy_bytes bytes each way) gives the factor from counter KiB to bytes;
+    traffic_out then receives one line per kernel: name, tail fetch bytes,
+    tail write bytes (calibrated)."""
     def load(path, counter):
+        rows = [r for r in csv.DictReader(open(path)) if r.get("Counter_Name", counter) == counter]
+        key = "Dispatch_Id" if rows and "Dispatch_Id" in rows[0] else None
+        if key:
+            rows.sort(key=lambda r: int(r[key]))
         vals = defaultdict(list)
-        for r in csv.DictReader(open(path)):
-            if r.get("Counter_Name", counter) != counter:
-                continue
+        for r in rows:
             vals[short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
         return vals
     f = load(fetch_csv, "FETCH_SIZE")
     w = load(write_csv, "WRITE_SIZE")
-    out = [f"{'kernel':34s} {'calls':>7s} {'FETCH_KiB':>11s} {'WRITE_KiB':>11s}"]
+    out = [f"{'kernel':34s} {'calls':>7s} {'FETCH_KiB':>11s} {'WRITE_KiB':>11s} {'tail_FETCH':>11s} "
+           f"{'tail_WRITE':>11s}"]
     for k in sorted(set(f) | set(w)):
         fv = statistics.mean(f[k]) if f.get(k) else float("nan")
         wv = statistics.mean(w[k]) if w.get(k) else float("nan")
-        out.append(f"{k:34s} {len(f.get(k, [])):7d} {fv:11.1f} {wv:11.1f}")
+        ft = statistics.mean(f[k][-tail:]) if f.get(k) else float("nan")
+        wt = statistics.mean(w[k][-tail:]) if w.get(k) else float("nan")
+        out.append(f"{k:34s} {len(f.get(k, [])):7d} {fv:11.1f} {wv:11.1f} {ft:11.1f} {wt:11.1f}")
     ck = short(calib_kernel)
     if copy_bytes and f.get(ck) and w.get(ck):
         kib = copy_bytes / 1024
@@ -66,7 +76,25 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_roccl
         fac_w = kib / max(w[ck])
         out.append(f"calibration ({ck}, largest dispatch = {kib:.0f} KiB each way): counter KiB x "
                    f"{fac_f:.4f} (fetch), x {fac_w:.4f} (write) = bytes / 1024")
+        if traffic_out:
+            with open(traffic_out, "w") as fo:
+                fo.write("# kernel  fetch_bytes  write_bytes  (mean of the last %d dispatches, calibrated)\n" % tail)
+                for k in sorted(set(f) & set(w)):
+                    fb = statistics.mean(f[k][-tail:]) * fac_f * 1024
+                    wb = statistics.mean(w[k][-tail:]) * fac_w * 1024
+                    fo.write(f"{k} {fb:.0f} {wb:.0f}\n")
     return "\n".join(out)
+
+
+def read_traffic(path):
+    """{kernel: (fetch_bytes, write_bytes)} from a pmc_summary traffic file."""
+    out = {}
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        name, fb, wb = line.rsplit(None, 2)
+        out[name] = (float(fb), float(wb))
+    return out
 
 
 def main():
@@ -80,11 +108,13 @@ def main():
     p.add_argument("write_csv")
     p.add_argument("--copy-bytes", type=float, default=None)
     p.add_argument("--calib-kernel", default="__amd_rocclr_copyBuffer")
+    p.add_argument("--tail", type=int, default=50)
+    p.add_argument("--traffic-out", default=None)
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace_summary(a.csv, a.tail))
     else:
-        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel))
+        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel, a.tail, a.traffic_out))
 
 
 if __name__ == "__main__":

@@ -109,19 +109,21 @@ class BatchSolver:
         if self.autotune:
             self.tune()
 
-    def set_launch(self, path="cell", block=256):
-        """Pins the stage path ("cell" / "face") and block size (64/128/256)."""
+    def set_launch(self, path="cell", block=256, xcd_order=True):
+        """Pins the stage path ("cell" / "face"), block size (64/128/256) and
+        block order (XCD-contiguous runs or the hardware's round robin)."""
         code = {"cell": L.STAGE_CELL, "face": L.STAGE_FACE}[path]
-        L.check(L.lib().esr_solver_set_launch(self.handle, code, int(block), stream_ptr()), "esr_solver_set_launch")
+        L.check(L.lib().esr_solver_set_launch(self.handle, code, int(block), int(bool(xcd_order)), stream_ptr()),
+                "esr_solver_set_launch")
 
     def tune(self, reps=10):
         """Autotuner: fastest stage path and block size for this solver's
         launch shape (cached per shape by the library)."""
-        p, b, us = C.c_int32(), C.c_int32(), C.c_double()
-        L.check(L.lib().esr_solver_autotune(self.handle, int(reps), C.byref(p), C.byref(b), C.byref(us),
-                                            stream_ptr()), "esr_solver_autotune")
+        p, b, x, us = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        L.check(L.lib().esr_solver_autotune(self.handle, int(reps), C.byref(p), C.byref(b), C.byref(x),
+                                            C.byref(us), stream_ptr()), "esr_solver_autotune")
         self.tuning = dict(path={L.STAGE_CELL: "cell", L.STAGE_FACE: "face"}[p.value], block=b.value,
-                           us_per_stage=us.value)
+                           xcd_order=bool(x.value), us_per_stage=us.value)
         return self.tuning
 
     def run(self, until_step=0):

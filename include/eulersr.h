@@ -265,17 +265,20 @@ int32_t esr_get_stage_path(void);
 
 /* Autotuner: times one SSP-RK2 stage of the prepared solver on its current
  * states for each stage path (cell, face) x block size (64, 128, 256 threads)
- * with HIP events, `reps` launches each, and keeps the fastest; the choice
- * is cached per launch shape (kernel configurations and work-item counts) for
- * the process, so later solvers of the same shape reuse it without timing.
- * Outputs (any may be null): the chosen path and block, and the device time
- * of one stage in microseconds. No counterpart in the reference. */
-esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, double* us_per_stage,
-                               void* stream);
+ * x block order (XCD-contiguous runs or plain) with HIP events, `reps`
+ * launches each, and keeps the fastest; the choice is cached per launch
+ * shape (kernel configurations and work-item counts) for the process, so
+ * later solvers of the same shape reuse it without timing. Outputs (any may
+ * be null): the chosen path, block and order, and the device time of one
+ * stage in microseconds. No counterpart in the reference. */
+esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32_t* block, int32_t* xcd_order,
+                               double* us_per_stage, void* stream);
 /* Pins the launch configuration of a solver: stage path (ESR_STAGE_CELL or
- * ESR_STAGE_FACE) and block size (64, 128 or 256). Results do not depend on
- * the block size. No counterpart in the reference. */
-esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, void* stream);
+ * ESR_STAGE_FACE), block size (64, 128 or 256) and block order (1: each XCD
+ * takes a contiguous run of blocks, 0: the hardware's round-robin order).
+ * Results do not depend on the block size or order. No counterpart in the
+ * reference. */
+esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, int32_t xcd_order, void* stream);
 const char* esr_last_error(void);
 int32_t esr_version(void);
 

@@ -39,15 +39,18 @@ def _run(specs, inits, launch=None, autotune=False, steps=40):
     return out, tuning
 
 
-def test_block_size_is_bit_invariant():
+def test_block_size_and_order_are_bit_invariant():
     specs, inits = _cases()
     for path in ("cell", "face"):
-        ref, _ = _run(specs, inits, launch=(path, 256))
-        for block in (64, 128):
-            print(f"path {path} block {block}", flush=True)
-            got, _ = _run(specs, inits, launch=(path, block))
-            for a, b in zip(got, ref):
-                assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        ref, _ = _run(specs, inits, launch=(path, 256, True))
+        for block in (64, 128, 256):
+            for xcd in (True, False):
+                if (block, xcd) == (256, True):
+                    continue
+                print(f"path {path} block {block} xcd order {xcd}", flush=True)
+                got, _ = _run(specs, inits, launch=(path, block, xcd))
+                for a, b in zip(got, ref):
+                    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
 def test_paths_agree_to_rounding():
@@ -65,7 +68,7 @@ def test_autotune_matches_pinned_choice_and_is_cached():
     print(f"autotune: {tuning}", flush=True)
     assert tuning["path"] in ("cell", "face") and tuning["block"] in (64, 128, 256)
     assert tuning["us_per_stage"] > 0
-    ref, _ = _run(specs, inits, launch=(tuning["path"], tuning["block"]))
+    ref, _ = _run(specs, inits, launch=(tuning["path"], tuning["block"], tuning["xcd_order"]))
     for a, b in zip(got, ref):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     # a second solver of the same launch shape reuses the cached choice
