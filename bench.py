@@ -93,13 +93,17 @@ TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profile
                             "traffic_bench_6case_h0.025.txt")
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, tuning):
     """HBM bytes per launch of `kernel` at the bench size from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
-    copy), or None."""
+    copy), when that summary was taken with the same launch configuration;
+    else None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    from eulersr_amd.profiling import read_traffic
+    from eulersr_amd.profiling import read_traffic, read_traffic_label
+    want = f"launch {tuning['path']},{tuning['block']},{int(bool(tuning['xcd_order']))}"
+    if read_traffic_label(TRAFFIC_FILE) != want:
+        return None
     t = read_traffic(TRAFFIC_FILE).get(PROFILE_NAMES.get(kernel, kernel))
     return None if t is None else t[0] + t[1]
 
@@ -222,12 +226,13 @@ def cd_drift(cases, stops, factor=2):
     return out
 
 
-def stage_kernels(cases, reps):
+def stage_kernels(cases, reps, launch=None):
     """Live HIP-event timing of the stage kernels on a batched solver of the
-    given HR cases (freestream states advanced 10 steps)."""
+    given HR cases (freestream states advanced 10 steps), autotuned or with a
+    pinned launch configuration."""
     from eulersr_amd.solve import BatchSolver
     bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3,
-                     autotune=True)
+                     autotune=launch is None, launch=launch)
     try:
         for i, c in enumerate(cases):
             bs.set_state(i, c.hr_init)
@@ -255,6 +260,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--scale-cases", type=int, default=256,
                     help="cases of the at-scale kernel measurement (0: skip)")
+    ap.add_argument("--launch", default=None,
+                    help="pin the stage launch of every solver instead of autotuning: path,block,xcd "
+                         "(e.g. cell,128,1)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: every solve capped at 2000 steps, 200 training steps, "
                          "no single-case pass, no CPU baseline")
@@ -274,6 +282,11 @@ def main():
     from eulersr_amd.device import require_gpu
     require_gpu()
     cap = args.max_steps if args.profile else 0
+    launch = None
+    if args.launch:
+        p_, b_, x_ = args.launch.split(",")
+        launch = (p_, int(b_), bool(int(x_)))
+        P.AUTOTUNE, P.LAUNCH = False, launch
     log(f"building meshes h={args.h}, LR h={args.h_lr}")
     meshes = build_meshes(args.h, args.h_lr)
     for g, (lm, hm) in meshes.items():
@@ -355,7 +368,7 @@ def main():
         drift = [abs(a - p["cd"]) for a, p in zip(later, single)]
         log(f"plain CD change from its stopping step to 2x that step: {['%.1e' % d for d in drift]}")
 
-    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps)
+    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, launch)
     log(f"autotuned launch of the 6-case batch: {tuning}")
     for name, k in kern.items():
         log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.2f} MB -> "
@@ -385,7 +398,7 @@ def main():
                                "and a 4x256 MLP trained on 55 solver-generated LR/HR pairs per geometry",
                    "cases": len(cases), "cells": cells, "h": args.h, "h_lr": args.h_lr},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, tuning),
                      "kernel": dom, "stage_path": path, "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,

@@ -34,6 +34,7 @@ CHECK_EVERY, PATIENCE = 5, 3     # eval/live_fvm.py:37-47
 THRESHOLD_MARGIN = 1.001         # eval/convergence_case.py:61-66
 ENGINEERING_TOL = 0.001          # eval/live_fvm.py:47
 AUTOTUNE = True                  # solvers pick their stage path / block size (esr_solver_autotune)
+LAUNCH = None                    # or a pinned (path, block, xcd_order) for every solver
 
 
 def ref_tf(geometry: str) -> float:
@@ -64,7 +65,7 @@ def fixed_time(specs, inits, max_steps=0):
     """Runs every case to its N = int(tf/dt)+1 (no early stop). Returns the
     reports (probe residual, CD, CL) and the final states (CUDA tensors)."""
     bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=CHECK_EVERY, patience=PATIENCE,
-                     max_steps=max_steps, autotune=AUTOTUNE)
+                     max_steps=max_steps, autotune=AUTOTUNE, launch=LAUNCH)
     try:
         for i, W in enumerate(inits):
             bs.set_state(i, W)
@@ -83,7 +84,7 @@ def converge(specs, inits, max_steps=0, sync=True):
     check every 5, patience 3) or N; returns (reports, states, solve seconds)."""
     import torch
     bs = BatchSolver(specs, scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY, patience=PATIENCE,
-                     max_steps=max_steps, autotune=AUTOTUNE)
+                     max_steps=max_steps, autotune=AUTOTUNE, launch=LAUNCH)
     try:
         for i, W in enumerate(inits):
             bs.set_state(i, W)

@@ -42,7 +42,7 @@ def trace_summary(path, tail=50):
 
 
 def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_rocclr_copyBuffer", tail=50,
-                traffic_out=None):
+                traffic_out=None, label=""):
     """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch): mean over all
     dispatches and over the last `tail` dispatches (bench.py ends with
     back-to-back launches of the stage kernels at the bench size). With
@@ -78,12 +78,20 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_roccl
                    f"{fac_f:.4f} (fetch), x {fac_w:.4f} (write) = bytes / 1024")
         if traffic_out:
             with open(traffic_out, "w") as fo:
+                fo.write(f"# {label}\n")
                 fo.write("# kernel  fetch_bytes  write_bytes  (mean of the last %d dispatches, calibrated)\n" % tail)
                 for k in sorted(set(f) & set(w)):
                     fb = statistics.mean(f[k][-tail:]) * fac_f * 1024
                     wb = statistics.mean(w[k][-tail:]) * fac_w * 1024
                     fo.write(f"{k} {fb:.0f} {wb:.0f}\n")
     return "\n".join(out)
+
+
+def read_traffic_label(path):
+    """First header line of a traffic file (the run's launch configuration)."""
+    with open(path) as f:
+        first = f.readline()
+    return first[1:].strip() if first.startswith("#") else ""
 
 
 def read_traffic(path):
@@ -110,11 +118,12 @@ def main():
     p.add_argument("--calib-kernel", default="__amd_rocclr_copyBuffer")
     p.add_argument("--tail", type=int, default=50)
     p.add_argument("--traffic-out", default=None)
+    p.add_argument("--label", default="")
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace_summary(a.csv, a.tail))
     else:
-        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel, a.tail, a.traffic_out))
+        print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel, a.tail, a.traffic_out, a.label))
 
 
 if __name__ == "__main__":

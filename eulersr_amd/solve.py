@@ -49,9 +49,10 @@ def get_stage_path() -> str:
 
 class BatchSolver:
     def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
-                 graph=True, autotune=False):
+                 graph=True, autotune=False, launch=None):
         require_gpu()
         self.autotune = bool(autotune)
+        self.launch = tuple(launch) if launch else None   # (path, block, xcd_order), applied at prepare
         self.tuning = None
         self.specs = list(cases)
         n = len(self.specs)
@@ -106,7 +107,11 @@ class BatchSolver:
 
     def prepare(self):
         L.check(L.lib().esr_solver_prepare(self.handle, stream_ptr()), "esr_solver_prepare")
-        if self.autotune:
+        if self.launch:
+            self.set_launch(*self.launch)
+            self.tuning = dict(path=self.launch[0], block=int(self.launch[1]), xcd_order=bool(self.launch[2]),
+                               us_per_stage=None)
+        elif self.autotune:
             self.tune()
 
     def set_launch(self, path="cell", block=256, xcd_order=True):
