@@ -1,31 +1,41 @@
-"""Benchmark: converged HR cases/hour of the batched learned warm start.
+"""Benchmark: converged HR cases/hour of the batched learned warm start,
+against the plain single-case GPU solver.
 
-Workload (one GPU): six HR cases -- a symmetric diamond (chord 1, half-angle
-5 deg) and a NACA0012 at h = 0.025, M in {0.9, 1.1, 2.0}, alpha = 2 deg --
-with HLLC flux, MUSCL reconstruction, SSP-RK2, CFL 0.6 (0.45 in the
-transonic band), FP32 state with FP64 residual norms and forces. Synthetic
-data: meshes from our own mesher, states generated by the solver.
+Workload (one GPU) -- the full form of the warm-start metric: a ragged batch
+of 256 HR cases at h = 0.025 over five geometries (a diamond with half-angle
+5 deg, NACA0012, NACA2412, NACA4412, NACA0015; 22k-34k cells each), with
+(M, alpha) drawn by default_rng(0) from M in [0.7, 3.0] (step 0.01) x alpha
+in [-5, 5] (step 0.5), half HLLC and half Roe flux, MUSCL reconstruction,
+SSP-RK2, CFL 0.6 (0.45 in the transonic band), FP32 state with FP64 residual
+norms and forces. Synthetic data: meshes from our own mesher, states
+generated by the solver.
 
 Protocol (eval/convergence_case.py:30-142, eval/live_fvm.py:37-47): each
 case's HR reference field is a fixed-time solve (tf = 5 for the diamond, 3
-otherwise, eval/live_fvm.py:21-22); 1.001 x its residual is the case's
-threshold; a solve has converged when the stationarity residual stays below
-it for 3 consecutive checks (every 5 steps). The reference fields, the
-solver-generated LR/HR training pairs (55 (M, alpha) per geometry, none of
-them a test case) and the training of one warm-start network per geometry
-are set-up work, outside the timed region (their times are reported).
+otherwise, eval/live_fvm.py:21-22); 1.001 x its residual (the larger of its
+one-step probe and its last 16 check residuals) is the case's threshold; a
+solve has converged when the stationarity residual stays below it for 3
+consecutive checks (every 5 steps), within a cap of 40,000 steps per case.
+Set-up outside the timed region (times reported): the reference fields, per
+geometry 55 solver-generated LR (h = 0.1) / HR training pairs (none of them
+a test case) and the training of one warm-start network per geometry.
 
-A "step" of this benchmark is one batched warm-start pipeline over the six
-cases: LR (h = 0.1) fixed-time solves, kNN/IDW features + MLP inference on
-the GPU, HR states, HR solves to the threshold. ``value`` = converged cases
-per hour over the K timed pipelines. Reported beside it: the plain
-single-case GPU solver (each case alone, cold start, same thresholds), the
-batched cold start, the batched IDW-only warm start, |dCD| of the warm start
-against the plain solve, and the stage kernels' HBM rates. The roofline
-object is for the dominant stage kernel, timed with HIP events on the
-solver's stream over back-to-back launches; its algorithmic bytes per launch
-are computed from the launch's work-items (``recon_bytes``, ``face_bytes``,
-``update_bytes``).
+A "step" of this benchmark is one warm-start campaign over the 256 cases:
+LR fixed-time solves of all cases in one solver, kNN/IDW features and MLP
+inference on the GPU, HR states with the freestream fallback, then the HR
+solves with continuous batching -- 64 cases in flight, each slot refilled
+with the next waiting case of its kind as soon as its case is done
+(esr_solver_replace_case). ``value`` = converged cases per hour over the K
+timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
+the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
+the warm start against those plain solves, the same campaign from cold
+starts, the stage kernels' HBM rates at a 256-case batch, and the
+reference's single-case configuration (6 cases: diamond + NACA0012, M in
+{0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
+for the slowest stage kernel of the autotuned launch at the 256-case batch,
+timed with HIP events on the solver's stream over back-to-back launches; its
+algorithmic bytes per launch come from the launch's work-items
+(``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -40,6 +50,7 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def log(msg):
@@ -78,34 +89,36 @@ def cell_bytes(items, cells, f_items, faces):
     return items * 80 + cells * 56
 
 
-KERNELS = (("k_recon", recon_bytes), ("k_face<MUSCL,HLLC>", face_bytes), ("k_update<stage1>", update_bytes),
-           ("k_cell<MUSCL,HLLC,stage1>", cell_bytes))
+KERNELS = (("k_recon", recon_bytes), ("k_face", face_bytes), ("k_update<stage1>", update_bytes),
+           ("k_cell<stage1>", cell_bytes))
 # kernels of one stage on each path
-PATH_KERNELS = {"cell": ("k_recon", "k_cell<MUSCL,HLLC,stage1>"),
-                "face": ("k_recon", "k_face<MUSCL,HLLC>", "k_update<stage1>")}
+PATH_KERNELS = {"cell": ("k_recon", "k_cell<stage1>"), "face": ("k_recon", "k_face", "k_update<stage1>")}
+# rocprofv3 name prefixes of the timed kernels (every flux / recon
+# instantiation of a launch set; update mode 2 = stage 1)
+PROFILE_PREFIX = {"k_recon": "k_recon", "k_face": "k_face<", "k_update<stage1>": "k_update<2>",
+                  "k_cell<stage1>": "k_cell<"}
 
 
-# rocprofv3 names of the timed kernels (template arguments: recon 1 = MUSCL,
-# flux 4 = HLLC, entropy 0; update mode 2 = stage 1)
-PROFILE_NAMES = {"k_recon": "k_recon", "k_face<MUSCL,HLLC>": "k_face<1, 4, 0>", "k_update<stage1>": "k_update<2>",
-                 "k_cell<MUSCL,HLLC,stage1>": "k_cell<1, 4, 0>"}
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                            "traffic_bench_6case_h0.025.txt")
+def traffic_file(tag):
+    return os.path.join(ROOT, "profiles", f"traffic_bench_{tag}.txt")
 
 
-def measured_traffic(kernel, tuning):
-    """HBM bytes per launch of `kernel` at the bench size from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a 1 GiB
-    copy), when that summary was taken with the same launch configuration;
-    else None."""
-    if not os.path.exists(TRAFFIC_FILE):
+def measured_traffic(kernel, tuning, tag):
+    """HBM bytes per launch of `kernel` (summed over the flux classes of the
+    launch set) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    of this workload (calibrated against a 1 GiB copy), when that summary was
+    taken with the same launch configuration; else None."""
+    path = traffic_file(tag)
+    if not os.path.exists(path):
         return None
     from eulersr_amd.profiling import read_traffic, read_traffic_label
     want = f"launch {tuning['path']},{tuning['block']},{int(bool(tuning['xcd_order']))}"
-    if read_traffic_label(TRAFFIC_FILE) != want:
+    if read_traffic_label(path) != want:
         return None
-    t = read_traffic(TRAFFIC_FILE).get(PROFILE_NAMES.get(kernel, kernel))
-    return None if t is None else t[0] + t[1]
+    pre = PROFILE_PREFIX[kernel]
+    hits = [fb + wb for name, (fb, wb) in read_traffic(path).items()
+            if (name.startswith(pre) if pre.endswith("<") else name == pre)]
+    return sum(hits) if hits else None
 
 
 def kernel_table(sizes, ms):
@@ -116,79 +129,73 @@ def kernel_table(sizes, ms):
     return kern
 
 
-TEST_MACHS, TEST_AOA = (0.9, 1.1, 2.0), 2.0
-# training grid of the warm-start network (the test cases are not in it)
+GEOMETRIES = ("diamond", "naca0012", "naca2412", "naca4412", "naca0015")
+# training grid of the warm-start networks (no test case is on it)
 TRAIN_MACHS = (0.7, 0.8, 0.95, 1.05, 1.2, 1.4, 1.6, 1.8, 2.2, 2.5, 2.8)
 TRAIN_AOAS = (-4.0, -1.0, 1.0, 3.0, 5.0)
-GEOMETRIES = ("diamond", "naca0012")
+SMALL_MACHS, SMALL_AOA = (0.9, 1.1, 2.0), 2.0
 
 
-def build_meshes(h, h_lr):
+def build_meshes(geoms, h, h_lr):
     from eulersr_amd.meshing import build_mesh
-    return {g: (build_mesh(g, h_lr), build_mesh(g, h)) for g in GEOMETRIES}
+    return {g: (build_mesh(g, h_lr), build_mesh(g, h)) for g in geoms}
 
 
-def scale_measurement(meshes, n_cases, steps=200, reps=20):
-    """Kernel throughput at batch scale: n_cases (M, alpha) cases drawn from
-    the M in [0.7, 3.0] x alpha in [-5, 5] grid with default_rng(0), split
-    over the two HR meshes, HLLC + MUSCL; times `steps` SSP-RK2 steps and the
-    stage kernels with HIP events."""
-    import torch
-    from eulersr_amd import main as M
+def ragged_grid(n, seed=0):
+    """(geometry, M, alpha, flux) of n cases: geometry round robin, flux
+    alternating HLLC / Roe per round, (M, alpha) from the reference's grid."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        m = round(0.70 + 0.01 * int(rng.integers(0, 231)), 2)
+        a = -5.0 + 0.5 * int(rng.integers(0, 21))
+        if a in TRAIN_AOAS and m in TRAIN_MACHS:      # keep test cases off the training grid
+            m = round(m + 0.01, 2)
+        out.append((GEOMETRIES[i % len(GEOMETRIES)], m, a, "HLLC" if (i // len(GEOMETRIES)) % 2 == 0 else "ROE"))
+    return out
+
+
+def make_workload(meshes, grid, cap_ref=0):
+    """WarmCase list (input order) with HR reference fields from fixed-time
+    solves (one batch for all cases)."""
+    from eulersr_amd import pipeline as P
+    lr_specs, lr_inits, hr_specs, hr_inits, keys = [], [], [], [], []
+    for g, m, a, fl in grid:
+        lm, hm = meshes[g]
+        ls, li = P.make_cases(lm, g, [(m, a)], flux=fl)
+        hs, hi = P.make_cases(hm, g, [(m, a)], flux=fl)
+        lr_specs += ls
+        lr_inits += li
+        hr_specs += hs
+        hr_inits += hi
+        keys.append((g, m, a, fl))
+    t0 = time.perf_counter()
+    ref, _ = P.fixed_time(hr_specs, hr_inits, max_steps=cap_ref)
+    t_ref = time.perf_counter() - t0
+    targets = P.with_targets(hr_specs, ref)
+    cases = [P.WarmCase(g, m, a, lr_specs[i], lr_inits[i], targets[i], hr_inits[i])
+             for i, (g, m, a, fl) in enumerate(keys)]
+    return cases, ref, t_ref
+
+
+def stage_kernels(cases, reps, launch=None):
+    """Live HIP-event timing of the stage kernels on a batched solver of the
+    given HR cases (freestream states advanced 10 steps), autotuned or with a
+    pinned launch configuration."""
     from eulersr_amd.solve import BatchSolver
-    rng = np.random.default_rng(0)
-    specs, inits = [], []
-    geoms = list(meshes)
-    for i in range(n_cases):
-        mach = 0.70 + 0.01 * int(rng.integers(0, 231))
-        aoa = -5.0 + 0.5 * int(rng.integers(0, 21))
-        g = geoms[i % len(geoms)]
-        cfg = dict(rho_inf=1.0, p_inf=1.0, gamma=1.4, Mach=mach, aoa=aoa, case=g, time_scheme="SRK2",
-                   flux="HLLC", reconstruction="MUSCL", CFL=0.6, tf=1.0e4)
-        W, inlet = M.initialize(meshes[g], cfg)
-        specs.append(M.case_spec(meshes[g], inlet, cfg))
-        inits.append(W)
-    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=3, autotune=True)
-    for i, W in enumerate(inits):
-        bs.set_state(i, W)
-    bs.prepare()
-    bs.run(10)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    bs.run(10 + steps)
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    sizes = bs.info()
-    kern = kernel_table(sizes, bs.time_kernels(reps))
-    tuning = bs.tuning
-    bs.close()
-    cells = sizes[0]
-    return dict(cases=n_cases, cell_items=cells, us_per_step=t / steps * 1e6,
-                cell_updates_per_s=cells * steps / t, stage_kernels=kern, tuning=tuning)
-
-
-def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
-    """numpy oracle (float32) SSP-RK2 steps on one HR mesh, 1 process."""
-    from oracle import fvm as F
-    om = F.omesh(mesh, np.float32)
-    W = np.asarray(spec.kw["value"], dtype=np.float32)[None, :].repeat(mesh.n_cells, 0)
-    kw = dict(spec.kw)
-    dt = F.get_dt(W, om, CFL=spec.cfl)
-    t0 = time.perf_counter()
-    W = F.time_step_RK2_SSP(W, om, dt, **kw)
-    one = time.perf_counter() - t0
-    n = max(1, int(target_s / max(one, 1e-3)))
-    t0 = time.perf_counter()
-    for _ in range(n):
-        W = F.time_step_RK2_SSP(W, om, dt, **kw)
-    t = time.perf_counter() - t0
-    cups = n * mesh.n_cells / t
-    cases_h = cups / (mesh.n_cells * steps_per_case) * 3600.0
-    return dict(value=cases_h, unit="cases/hour", cores=1, kind="port",
-                sample=(f"numpy oracle float32, {n} SSP-RK2 steps on diamond h=0.025 ({mesh.n_cells} cells) "
-                        f"in {t:.1f}s = {cups:.3g} cell-updates/s; cases/hour extrapolated with the GPU-measured "
-                        f"mean {steps_per_case:.0f} steps per converged case"),
-                cell_updates_per_s=cups)
+    bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3,
+                     autotune=launch is None, launch=launch)
+    try:
+        for i, c in enumerate(cases):
+            bs.set_state(i, c.hr_init)
+        bs.prepare()
+        bs.run(10)
+        sizes = bs.info()
+        ms = bs.time_kernels(reps)
+        tuning = bs.tuning
+    finally:
+        bs.close()
+    return kernel_table(sizes, ms), sizes, tuning
 
 
 def plain_single(cases, max_steps):
@@ -218,32 +225,76 @@ def cd_drift(cases, stops, factor=2):
             bs.set_state(i, c.hr_init)
         bs.prepare()
         for i in sorted(range(len(cases)), key=lambda i: stops[i]):
-            target = int(factor * stops[i]) // 5 * 5
-            bs.run(target)
+            bs.run(int(factor * stops[i]) // 5 * 5)
             out[i] = bs.report(i)["cd"]
     finally:
         bs.close()
     return out
 
 
-def stage_kernels(cases, reps, launch=None):
-    """Live HIP-event timing of the stage kernels on a batched solver of the
-    given HR cases (freestream states advanced 10 steps), autotuned or with a
-    pinned launch configuration."""
-    from eulersr_amd.solve import BatchSolver
-    bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3,
-                     autotune=launch is None, launch=launch)
-    try:
-        for i, c in enumerate(cases):
-            bs.set_state(i, c.hr_init)
-        bs.prepare()
-        bs.run(10)
-        sizes = bs.info()
-        ms = bs.time_kernels(reps)
-        tuning = bs.tuning
-    finally:
-        bs.close()
-    return kernel_table(sizes, ms), sizes, tuning
+def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
+    """numpy oracle (float32) SSP-RK2 steps on one HR mesh, 1 process."""
+    from oracle import fvm as F
+    om = F.omesh(mesh, np.float32)
+    W = np.asarray(spec.kw["value"], dtype=np.float32)[None, :].repeat(mesh.n_cells, 0)
+    kw = dict(spec.kw)
+    dt = F.get_dt(W, om, CFL=spec.cfl)
+    t0 = time.perf_counter()
+    W = F.time_step_RK2_SSP(W, om, dt, **kw)
+    one = time.perf_counter() - t0
+    n = max(1, int(target_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        W = F.time_step_RK2_SSP(W, om, dt, **kw)
+    t = time.perf_counter() - t0
+    cups = n * mesh.n_cells / t
+    cases_h = cups / (mesh.n_cells * steps_per_case) * 3600.0
+    return dict(value=cases_h, unit="cases/hour", cores=1, kind="port",
+                sample=(f"numpy oracle float32, {n} SSP-RK2 steps on diamond h=0.025 ({mesh.n_cells} cells) "
+                        f"in {t:.1f}s = {cups:.3g} cell-updates/s; cases/hour extrapolated with the GPU-measured "
+                        f"mean {steps_per_case:.0f} steps per converged plain case"),
+                cell_updates_per_s=cups)
+
+
+def rate(run):
+    return sum(x["converged"] for x in run["reports"]) / run["total_s"] * 3600.0
+
+
+def small_config(meshes, starter, args):
+    """The reference's single-case GPU configuration (6 cases) measured the
+    same way: batched warm start, batched cold start, plain single-case."""
+    from eulersr_amd import pipeline as P
+    cases = []
+    for g in ("diamond", "naca0012"):
+        lm, hm = meshes[g]
+        grid = [(m, SMALL_AOA) for m in SMALL_MACHS]
+        hs, hi = P.make_cases(hm, g, grid)
+        ref, _ = P.fixed_time(hs, hi)
+        cases += P.warm_cases(g, lm, hm, grid, ref)
+    starter.run(cases, mode="mlp", max_steps=20)
+    warm = starter.run(cases, mode="mlp", max_steps=args.max_steps)
+    cold = starter.run(cases, mode="cold", max_steps=args.max_steps)
+    idw = starter.run(cases, mode="idw", max_steps=args.max_steps)
+    st, sc, single = plain_single(cases, args.max_steps)
+    single_rate = sc / st * 3600.0 if st > 0 else 0.0
+    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, P.LAUNCH)
+    dom = max(PATH_KERNELS[tuning["path"]], key=lambda k: kern[k]["us_per_launch"])
+    out = dict(workload="diamond + NACA0012 at h=0.025, M in {0.9,1.1,2.0}, aoa 2deg, HLLC+MUSCL, SSP-RK2",
+               warm_cases_per_hour=rate(warm), cold_batched_cases_per_hour=rate(cold),
+               idw_batched_cases_per_hour=rate(idw), plain_single_case_cases_per_hour=single_rate,
+               speedup_vs_plain_single_case=rate(warm) / single_rate if single_rate else None,
+               pipeline_split_s={"lr": warm["lr_s"], "inference": warm["infer_s"], "hr": warm["hr_s"]},
+               steps_per_case={"mlp": [x["stopping_step"] for x in warm["reports"]],
+                               "idw": [x["stopping_step"] for x in idw["reports"]],
+                               "plain_single": [x["stopping_step"] for x in single]},
+               abs_dcd_vs_plain=[abs(w["cd"] - p["cd"]) for w, p in zip(warm["reports"], single)],
+               tuning=tuning, stage_kernels=kern, dominant_kernel=dom,
+               dominant_traffic=measured_traffic(dom, tuning, "6case_h0.025"),
+               launch_sizes=dict(zip(("cell_items", "group_cells", "face_items", "group_faces"), sizes)))
+    log(f"6-case config: warm {out['warm_cases_per_hour']:.0f}/h, plain single {single_rate:.0f}/h "
+        f"(x{out['speedup_vs_plain_single_case']:.2f}), steps {out['steps_per_case']['mlp']}, "
+        f"{dom} {kern[dom]['us_per_launch']:.1f} us = {kern[dom]['gbs']:.0f} GB/s")
+    return out
 
 
 def main():
@@ -251,26 +302,30 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--cases", type=int, default=256, help="cases of the ragged warm-start batch")
+    ap.add_argument("--slots", type=int, default=64, help="cases in flight (continuous batching)")
+    ap.add_argument("--single-sample", type=int, default=16,
+                    help="cases of the batch run alone by the plain single-case solver")
     ap.add_argument("--h", type=float, default=0.025)
     ap.add_argument("--h-lr", type=float, default=0.1)
-    ap.add_argument("--max-steps", type=int, default=100000)
+    ap.add_argument("--max-steps", type=int, default=40000)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--train-batch", type=int, default=16384)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-small", action="store_true", help="skip the 6-case single-case configuration")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--scale-cases", type=int, default=256,
-                    help="cases of the at-scale kernel measurement (0: skip)")
     ap.add_argument("--launch", default=None,
                     help="pin the stage launch of every solver instead of autotuning: path,block,xcd "
-                         "(e.g. cell,128,1)")
+                         "(e.g. face,64,0)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: every solve capped at 2000 steps, 200 training steps, "
-                         "no single-case pass, no CPU baseline")
+                         "no single-case passes, no cold batch, no 6-case config, no CPU baseline")
     args = ap.parse_args()
     if args.profile:
         args.max_steps = min(args.max_steps, 2000)
         args.train_steps = min(args.train_steps, 200)
         args.no_cpu_baseline = True
+        args.no_small = True
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the cases of a batch are independent and would be sharded "
@@ -282,31 +337,16 @@ def main():
     from eulersr_amd.device import require_gpu
     require_gpu()
     cap = args.max_steps if args.profile else 0
-    launch = None
     if args.launch:
         p_, b_, x_ = args.launch.split(",")
-        launch = (p_, int(b_), bool(int(x_)))
-        P.AUTOTUNE, P.LAUNCH = False, launch
+        P.AUTOTUNE, P.LAUNCH = False, (p_, int(b_), bool(int(x_)))
     log(f"building meshes h={args.h}, LR h={args.h_lr}")
-    meshes = build_meshes(args.h, args.h_lr)
+    meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
     for g, (lm, hm) in meshes.items():
         log(f"  {g}: LR {lm.n_cells} cells, HR {hm.n_cells} cells")
-    test_grid = [(m, TEST_AOA) for m in TEST_MACHS]
-    train_grid = [(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
-
-    # HR reference fields of the test cases (fixed-time solves): thresholds
-    # and CD/CL targets -- set-up, outside every timed region
-    log("HR reference fields (fixed-time solves)")
-    cases = []
-    for g, (lm, hm) in meshes.items():
-        hs, hi = P.make_cases(hm, g, test_grid)
-        ref, _ = P.fixed_time(hs, hi, max_steps=cap)
-        for (m, a), r in zip(test_grid, ref):
-            log(f"  {g} M={m}: N={r['N']} probe={r['final_residual']:.3e} "
-                f"last-checks max={r['recent_rel_max']:.3e} cd={r['cd']:.6f} cl={r['cl']:.6f}")
-        cases += P.warm_cases(g, lm, hm, test_grid, ref)
 
     # warm-start network per geometry, trained on solver-generated pairs
+    train_grid = [(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
     starter = P.WarmStarter()
     training = {}
     for g, (lm, hm) in meshes.items():
@@ -317,8 +357,7 @@ def main():
         t_data = time.perf_counter() - t0
         kmap = P.WS.KnnMap(lm, hm)
         net, info = P.train(kmap, pairs, steps=args.train_steps, batch=args.train_batch, seed=0)
-        info["data_s"] = t_data
-        info["cases"] = len(train_grid)
+        info.update(data_s=t_data, cases=len(train_grid))
         training[g] = info
         log(f"  {g}: {len(train_grid)} LR/HR pairs in {t_data:.1f}s; trained {info['steps']} steps in "
             f"{info['train_s']:.1f}s, loss {info['loss_first']:.3e} -> {info['loss_last']:.3e} "
@@ -326,108 +365,120 @@ def main():
         del pairs
         starter.add(g, lm, hm, net=net, kmap=kmap)
 
+    # the ragged batch and its HR reference fields (set-up)
+    grid = ragged_grid(args.cases)
+    log(f"ragged batch: {args.cases} cases; HR reference fields (fixed-time solves)")
+    cases, ref, t_ref = make_workload(meshes, grid, cap_ref=cap)
+    log(f"  reference fields in {t_ref:.1f}s")
+    cells = [c.hr_spec.mesh.n_cells for c in cases]
+
     # load code objects outside the timed region
-    for mode in ("cold", "mlp"):
-        starter.run(cases, mode=mode, max_steps=20)
+    starter.campaign(cases[:8], mode="mlp", slots=4, max_steps=20, lr_max_steps=20)
     for w in range(args.warmup):
-        log(f"warmup pipeline {w + 1}/{args.warmup}")
-        starter.run(cases, mode="mlp", max_steps=args.max_steps)
+        log(f"warmup campaign {w + 1}/{args.warmup}")
+        starter.campaign(cases, mode="mlp", slots=args.slots, max_steps=args.max_steps)
     times, runs = [], []
     for k in range(args.steps):
-        r = starter.run(cases, mode="mlp", max_steps=args.max_steps)
+        r = starter.campaign(cases, mode="mlp", slots=args.slots, max_steps=args.max_steps)
         times.append(r["total_s"])
         runs.append(r)
         conv = sum(x["converged"] for x in r["reports"])
-        log(f"timed warm-start pipeline {k + 1}/{args.steps}: {r['total_s']:.3f}s (LR {r['lr_s']:.3f}s, "
-            f"inference {r['infer_s'] * 1e3:.1f}ms, HR {r['hr_s']:.3f}s) converged={conv}/{len(cases)} "
-            f"steps={[x['stopping_step'] for x in r['reports']]} fallback={sum(r['fallback'])}")
+        st = [x["stopping_step"] for x in r["reports"] if x["converged"]]
+        log(f"timed warm-start campaign {k + 1}/{args.steps}: {r['total_s']:.2f}s (LR {r['lr_s']:.2f}s, "
+            f"inference {r['infer_s'] * 1e3:.1f}ms, HR {r['hr_s']:.2f}s, {args.slots} slots) "
+            f"converged={conv}/{len(cases)} steps median {np.median(st) if st else 0:.0f} "
+            f"mean {np.mean(st) if st else 0:.0f} max {max(st) if st else 0} fallback={sum(r['fallback'])}")
     T = sum(times)
     n_conv = sum(sum(x["converged"] for x in r["reports"]) for r in runs)
     value = n_conv / T * 3600.0
     warm = runs[-1]
 
-    log("batched cold start and batched IDW warm start")
-    cold = starter.run(cases, mode="cold", max_steps=args.max_steps)
-    idw = starter.run(cases, mode="idw", max_steps=args.max_steps)
-    rate = lambda r: sum(x["converged"] for x in r["reports"]) / r["total_s"] * 3600.0
-    log(f"  cold batched: {cold['total_s']:.3f}s steps={[x['stopping_step'] for x in cold['reports']]}")
-    log(f"  IDW  batched: {idw['total_s']:.3f}s steps={[x['stopping_step'] for x in idw['reports']]}")
+    # plain single-case GPU solver on a sample of the batch (cold, one at a time)
+    sample = list(range(min(args.single_sample, len(cases))))
     single_t, single_conv, single = 0.0, 0, None
     if not args.profile:
-        single_t, single_conv, single = plain_single(cases, args.max_steps)
+        single_t, single_conv, single = plain_single([cases[i] for i in sample], args.max_steps)
     single_rate = single_conv / single_t * 3600.0 if single_t > 0 else 0.0
-    log(f"plain single-case: {single_t:.2f}s, {single_conv} converged -> {single_rate:.0f} cases/hour")
-    plain = single or cold["reports"]
-    dcd = [abs(w["cd"] - p["cd"]) for w, p in zip(warm["reports"], plain)]
-    dcl = [abs(w["cl"] - p["cl"]) for w, p in zip(warm["reports"], plain)]
-    dcd_idw = [abs(w["cd"] - p["cd"]) for w, p in zip(idw["reports"], plain)]
-    log(f"|dCD| warm vs plain: {['%.1e' % d for d in dcd]}")
-    drift = None
+    log(f"plain single-case ({len(sample)} cases): {single_t:.2f}s, {single_conv} converged -> "
+        f"{single_rate:.0f} cases/hour")
+    dcd = dcl = drift = None
     if single is not None:
-        later = cd_drift(cases, [x["stopping_step"] for x in single])
+        dcd = [abs(warm["reports"][i]["cd"] - p["cd"]) for i, p in zip(sample, single)]
+        dcl = [abs(warm["reports"][i]["cl"] - p["cl"]) for i, p in zip(sample, single)]
+        later = cd_drift([cases[i] for i in sample], [p["stopping_step"] for p in single])
         drift = [abs(a - p["cd"]) for a, p in zip(later, single)]
-        log(f"plain CD change from its stopping step to 2x that step: {['%.1e' % d for d in drift]}")
+        log(f"|dCD| warm vs plain: median {np.median(dcd):.1e}, within 1e-4: {sum(d <= 1e-4 for d in dcd)}/"
+            f"{len(dcd)}; plain CD change to 2x its steps: median {np.median(drift):.1e}")
+    cold = None
+    if not args.profile:
+        cold = starter.campaign(cases, mode="cold", slots=args.slots, max_steps=args.max_steps)
+        cst = [x["stopping_step"] for x in cold["reports"] if x["converged"]]
+        log(f"cold-start campaign: {cold['total_s']:.2f}s -> {rate(cold):.0f} cases/hour, steps median "
+            f"{np.median(cst) if cst else 0:.0f} mean {np.mean(cst) if cst else 0:.0f}")
 
-    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, launch)
-    log(f"autotuned launch of the 6-case batch: {tuning}")
+    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, P.LAUNCH)
+    log(f"autotuned launch of the batch: {tuning}")
     for name, k in kern.items():
-        log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.2f} MB -> "
+        log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.1f} MB -> "
             f"{k['gbs']:.0f} GB/s")
-    path = tuning["path"]
-    dom = max(PATH_KERNELS[path], key=lambda k: kern[k]["us_per_launch"])
-    scale = None
-    if args.scale_cases > 0 and not args.profile:
-        log(f"at-scale kernel measurement: {args.scale_cases} cases")
-        scale = scale_measurement({g: hm for g, (lm, hm) in meshes.items()}, args.scale_cases)
-        for k, v in scale["stage_kernels"].items():
-            log(f"  {k}: {v['us_per_launch']:.1f} us/launch -> {v['gbs']:.0f} GB/s")
-        log(f"  {scale['us_per_step']:.1f} us/step, {scale['cell_updates_per_s']:.3g} cell-updates/s, "
-            f"autotuned {scale['tuning']}")
+    dom = max(PATH_KERNELS[tuning["path"]], key=lambda k: kern[k]["us_per_launch"])
+    cell_updates = sum(c * (x["stopping_step"] if x["converged"] else x["steps_run"])
+                       for c, x in zip(cells, warm["reports"]))
 
-    steps_case = [x["stopping_step"] if x["converged"] else x["steps_run"] for x in warm["reports"]]
-    cells = [c.hr_spec.mesh.n_cells for c in cases]
+    small = None
+    if not args.no_small:
+        small = small_config(meshes, starter, args)
+
     out = {
         "metric": "converged HR cases/hour, batched learned warm start (LR solve + BF16 MLP inference + HR "
                   "solve to the reference-field threshold, check 5, patience 3)",
         "value": value, "unit": "cases/hour", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": T / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"6 HR cases: diamond (half-angle 5deg) + NACA0012 at h={args.h}, "
-                               "M in {0.9,1.1,2.0}, aoa 2deg, HLLC+MUSCL, SSP-RK2, CFL 0.6 (0.45 transonic); "
-                               f"warm start from an LR h={args.h_lr} fixed-time solve through kNN(6)/IDW features "
-                               "and a 4x256 MLP trained on 55 solver-generated LR/HR pairs per geometry",
-                   "cases": len(cases), "cells": cells, "h": args.h, "h_lr": args.h_lr},
+        "config": {"workload": f"ragged batch of {args.cases} HR cases at h={args.h} over 5 geometries "
+                               "(diamond, NACA0012/2412/4412/0015), (M, aoa) from M in [0.7,3.0] x aoa in [-5,5], "
+                               "half HLLC half Roe, MUSCL, SSP-RK2, CFL 0.6 (0.45 transonic); warm start from an "
+                               f"LR h={args.h_lr} fixed-time solve through kNN(6)/IDW features and a 4x256 BF16 "
+                               "MLP per geometry trained on 55 solver-generated LR/HR pairs",
+                   "cases": args.cases, "h": args.h, "h_lr": args.h_lr, "cells_total": int(sum(cells))},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, tuning),
-                     "kernel": dom, "stage_path": path, "bytes_per_launch": kern[dom]["bytes_per_launch"],
+                     "frac": kern[dom]["gbs"] / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, tuning, f"{args.cases}case_h{args.h}"),
+                     "kernel": dom, "stage_path": tuning["path"], "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,
         "plain_single_case_cases_per_hour": single_rate,
-        "cold_batched_cases_per_hour": rate(cold),
-        "idw_batched_cases_per_hour": rate(idw),
+        "plain_single_case_sample": len(sample),
+        "cold_campaign_cases_per_hour": rate(cold) if cold else None,
+        "slots": args.slots,
+        "cell_updates_per_s": cell_updates / warm["hr_s"],
+        "converged": int(sum(x["converged"] for x in warm["reports"])),
+        "fallback": int(sum(warm["fallback"])),
         "pipeline_split_s": {"lr": warm["lr_s"], "inference": warm["infer_s"], "hr": warm["hr_s"]},
-        "steps_per_case": {"mlp": steps_case,
-                           "idw": [x["stopping_step"] for x in idw["reports"]],
-                           "cold_batched": [x["stopping_step"] for x in cold["reports"]],
-                           "plain_single": [x["stopping_step"] for x in plain]},
-        "converged": {"mlp": [bool(x["converged"]) for x in warm["reports"]],
-                      "plain_single": [bool(x["converged"]) for x in plain]},
-        "fallback": warm["fallback"],
-        "abs_dcd_vs_plain": dcd, "abs_dcl_vs_plain": dcl, "abs_dcd_idw_vs_plain": dcd_idw,
-        "dcd_within_1e-4": sum(d <= 1e-4 for d in dcd),
+        "steps_warm": {"median": float(np.median([x["stopping_step"] for x in warm["reports"]])),
+                       "mean": float(np.mean([x["stopping_step"] for x in warm["reports"]])),
+                       "max": int(max(x["stopping_step"] for x in warm["reports"]))},
+        "steps_cold": None if not cold else {
+            "median": float(np.median([x["stopping_step"] for x in cold["reports"]])),
+            "mean": float(np.mean([x["stopping_step"] for x in cold["reports"]])),
+            "max": int(max(x["stopping_step"] for x in cold["reports"]))},
+        "steps_plain_sample": [p["stopping_step"] for p in single] if single else None,
+        "steps_warm_sample": [warm["reports"][i]["stopping_step"] for i in sample],
+        "abs_dcd_vs_plain": dcd, "abs_dcl_vs_plain": dcl,
+        "dcd_within_1e-4": None if dcd is None else int(sum(d <= 1e-4 for d in dcd)),
         "plain_cd_change_to_2x_steps": drift,
-        "thresholds": [c.hr_spec.threshold for c in cases],
+        "reference_fields_s": t_ref,
         "training": training,
-        "stage_kernels": kern,
         "tuning": tuning,
-        "at_scale": scale,
+        "stage_kernels": kern,
         "launch_sizes": dict(zip(("cell_items", "group_cells", "face_items", "group_faces"), sizes)),
+        "single_case_config": small,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (numpy oracle)")
-        conv_steps = [s for s, x in zip(out["steps_per_case"]["plain_single"], plain) if x["converged"]]
-        mean_steps = float(np.mean(conv_steps or out["steps_per_case"]["plain_single"]))
-        dia = [c for c in cases if c.geometry == "diamond"][-1]
+        conv_steps = [p["stopping_step"] for p in (single or []) if p["converged"]]
+        mean_steps = float(np.mean(conv_steps)) if conv_steps else 15000.0
+        dia = next(c for c in cases if c.geometry == "diamond")
         out["cpu_baseline"] = cpu_baseline(meshes["diamond"][1], dia.hr_spec, mean_steps)
     if args.profile:
         # counter calibration: a plain device copy of known size (1 GiB,

@@ -45,6 +45,7 @@ class Case(C.Structure):
         ("q_ref", C.c_double), ("track_engineering", C.c_int32),
         ("target_cd", C.c_double), ("target_cl", C.c_double), ("cd_tol_abs", C.c_double),
         ("cl_tol_abs", C.c_double), ("engineering_patience", C.c_int32), ("threshold", C.c_double),
+        ("max_steps", C.c_int64),
     ]
 
 
@@ -98,6 +99,9 @@ SIGNATURES = {
     "esr_solver_prepare": (C.c_int, [C.c_void_p, C.c_void_p]),
     "esr_solver_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]),
     "esr_solver_finalize": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "esr_solver_replace_case": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Case), C.c_void_p, C.c_void_p]),
+    "esr_solver_reports": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CaseReport)]),
+    "esr_solver_compact": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.c_void_p]),
     "esr_solver_report": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(CaseReport)]),
     "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),

@@ -111,6 +111,8 @@ struct Engine {
   int32_t n_cases = 0;
   int scheme = ESR_SCHEME_SRK2;
   int32_t stage_path = ESR_STAGE_CELL;
+  int32_t cur_block = BLOCK, cur_xcd = 1;
+  std::vector<uint8_t> group_on;      // empty: every group launched
   std::vector<GroupDev> groups;       // host copies
   std::vector<const esr_mesh*> gmesh; // mesh of each group
   std::vector<int32_t> case_group;    // global case -> group
@@ -141,6 +143,7 @@ struct Engine {
   void block_tables(Cls& c, int32_t block, int32_t xcd = 1) {
     std::vector<int32_t> blk, fblk;
     for (int32_t gi : c.gids) {
+      if (!group_on.empty() && !group_on[gi]) continue;  // every case of the group is done
       GroupDev& g = groups[gi];
       const int64_t items = (int64_t)g.m.n * g.C;
       g.blk0 = (int32_t)blk.size();
@@ -172,7 +175,19 @@ struct Engine {
     hip_check(hipMemcpy(groups_d.p, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice),
               "groups H2D");
     stage_path = path;
+    cur_block = block;
+    cur_xcd = xcd;
   }
+
+  // launches cover only the groups with mask[g] = 1 (continuous batching:
+  // a group whose cases are all done is skipped); true when it changed
+  bool set_group_mask(const std::vector<uint8_t>& mask, hipStream_t st) {
+    if (mask == group_on) return false;
+    group_on = mask;
+    configure(stage_path, cur_block, cur_xcd, st);
+    return true;
+  }
+  bool on(size_t gi) const { return group_on.empty() || group_on[gi]; }
 
   float4* buf(int which) {
     return which == 0 ? W0.as<float4>() : which == 1 ? W1.as<float4>() : W2.as<float4>();
@@ -267,13 +282,7 @@ struct Engine {
     cases_h.assign(n_cases, CaseDev{});
     for (int32_t gc = 0; gc < n_cases; ++gc) {
       const CaseSpec& s = specs[user_of_global[gc]];
-      CaseDev& cd = cases_h[gc];
-      cd.value = make_float4(s.opts.value[0], s.opts.value[1], s.opts.value[2], s.opts.value[3]);
-      const esr_mesh* m = s.mesh;
-      const float width = s.opts.sponge_width > 0 ? s.opts.sponge_width : 0.2f * (m->y_max - m->y_min);
-      cd.strength = s.opts.sponge_strength > 0 ? s.opts.sponge_strength : sponge_strength(s.opts, width);
-      cd.dt = cd.dt_half = cd.dt_sixth = 0.f;
-      cd.active = 1;
+      cases_h[gc] = case_params(s.mesh, s.opts);
     }
     upload(cases_d, cases_h);
     global_of_user.assign(n_cases, 0);
@@ -360,6 +369,30 @@ struct Engine {
   }
 
   // per-case dt minimum of W0 (times cfl) -> device dt params and dt_d
+  // parameters of a case from its options (dt set later)
+  static CaseDev case_params(const esr_mesh* m, const esr_opts& o) {
+    CaseDev cd{};
+    cd.value = make_float4(o.value[0], o.value[1], o.value[2], o.value[3]);
+    const float width = o.sponge_width > 0 ? o.sponge_width : 0.2f * (m->y_max - m->y_min);
+    cd.strength = o.sponge_strength > 0 ? o.sponge_strength : sponge_strength(o, width);
+    cd.dt = cd.dt_half = cd.dt_sixth = 0.f;
+    cd.active = 1;
+    return cd;
+  }
+
+  // dt of global case gc alone from its current state (the other cases keep theirs)
+  float compute_dt_one(int32_t gc, float cfl, hipStream_t st) {
+    hip_check(hipMemcpyAsync(cfl_d.as<float>() + gc, &cfl, sizeof(float), hipMemcpyHostToDevice, st), "cfl H2D");
+    const int32_t gi = case_group[gc];
+    launch_dt(groups[gi], gdev(gi), cases(), W0.as<float4>(), part_f.as<float>(), rstride, 0, st);
+    launch_dt_final(1, cases() + gc, part_f.as<float>() + (int64_t)gc * rstride, nranges.as<int32_t>() + gc, rstride,
+                    cfl_d.as<float>() + gc, dt_d.as<float>() + gc, st);
+    float dt = 0.f;
+    hip_check(hipMemcpyAsync(&dt, dt_d.as<float>() + gc, sizeof(float), hipMemcpyDeviceToHost, st), "dt D2H");
+    hip_check(hipStreamSynchronize(st), "compute_dt_one sync");
+    return dt;
+  }
+
   void compute_dt(const std::vector<float>& cfl_user, hipStream_t st) {
     std::vector<float> cfl(n_cases);
     for (int32_t u = 0; u < n_cases; ++u) cfl[global_of_user[u]] = cfl_user[u];
@@ -374,13 +407,17 @@ struct Engine {
   // norms of (Wn - Wref) and Wref for cases flagged 'check'
   // fused: the check arguments, run by the last block of the last launch
   void norms(const float4* Wn, float4* Wref_, int copy_ref, hipStream_t st, const CheckArgs* fused = nullptr) {
+    size_t last = groups.size();
     for (size_t gi = 0; gi < groups.size(); ++gi)
-      launch_norms(groups[gi], gdev((int32_t)gi), cases(), Wn, Wref_, part_d.as<double>(), part_r.as<double>(),
-                   rstride, copy_ref, st, gi + 1 == groups.size() ? fused : nullptr);
+      if (on(gi)) last = gi;
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      if (on(gi))
+        launch_norms(groups[gi], gdev((int32_t)gi), cases(), Wn, Wref_, part_d.as<double>(), part_r.as<double>(),
+                     rstride, copy_ref, st, gi == last ? fused : nullptr);
   }
   void forces(const float4* W, hipStream_t st) {
     for (size_t gi = 0; gi < groups.size(); ++gi)
-      launch_forces(groups[gi], gdev((int32_t)gi), cases(), W, forces_d.as<double>(), st);
+      if (on(gi)) launch_forces(groups[gi], gdev((int32_t)gi), cases(), W, forces_d.as<double>(), st);
   }
 };
 
@@ -495,29 +532,34 @@ struct esr_solver {
 
 namespace {
 
+// loop state of user case u entering its slot at solver step `start`
+// (s->N[u] is the absolute step N of the case)
+CaseLoop initial_loop(const esr_solver* s, int32_t u, int64_t start) {
+  const esr_case& c = s->cases[u];
+  CaseLoop l;
+  std::memset(&l, 0, sizeof(l));
+  l.N = s->N[u];
+  l.start = start;
+  l.ref_step = start;
+  l.stopping_step = s->N[u];
+  l.last_rel = INFINITY;
+  l.threshold = c.threshold > 0 ? c.threshold : s->so.threshold;
+  l.patience = std::max(1, s->so.patience);
+  l.q_ref = c.q_ref;
+  l.track_eng = c.track_engineering ? 1 : 0;
+  l.target_cd = c.target_cd;
+  l.target_cl = c.target_cl;
+  l.cd_tol = c.cd_tol_abs;
+  l.cl_tol = c.cl_tol_abs;
+  l.eng_patience = std::max(1, c.engineering_patience);
+  l.eng_stopping_step = s->N[u];
+  return l;
+}
+
 void solver_set_loops(esr_solver* s, hipStream_t st) {
   Engine& e = s->eng;
   std::vector<CaseLoop> L(e.n_cases);
-  for (int32_t u = 0; u < e.n_cases; ++u) {
-    const int32_t gc = e.global_of_user[u];
-    const esr_case& c = s->cases[u];
-    CaseLoop& l = L[gc];
-    std::memset(&l, 0, sizeof(l));
-    l.N = s->N[u];
-    l.ref_step = 0;
-    l.stopping_step = s->N[u];
-    l.last_rel = INFINITY;
-    l.threshold = c.threshold > 0 ? c.threshold : s->so.threshold;
-    l.patience = std::max(1, s->so.patience);
-    l.q_ref = c.q_ref;
-    l.track_eng = c.track_engineering ? 1 : 0;
-    l.target_cd = c.target_cd;
-    l.target_cl = c.target_cl;
-    l.cd_tol = c.cd_tol_abs;
-    l.cl_tol = c.cl_tol_abs;
-    l.eng_patience = std::max(1, c.engineering_patience);
-    l.eng_stopping_step = s->N[u];
-  }
+  for (int32_t u = 0; u < e.n_cases; ++u) L[e.global_of_user[u]] = initial_loop(s, u, 0);
   e.loops_d.alloc(L.size() * sizeof(CaseLoop));
   hip_check(hipMemcpyAsync(e.loops_d.p, L.data(), L.size() * sizeof(CaseLoop), hipMemcpyHostToDevice, st),
             "loops H2D");
@@ -751,8 +793,13 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream) {
       const float q = s->cases[u].tf / dt;  // float32 division, as tf / dt in the reference
       if (!(q >= 0.f) || !std::isfinite(q)) throw Error(ESR_ERR_VALUE, "non-finite time step");
       s->N[u] = (int64_t)q + 1;
+      if (s->cases[u].max_steps > 0) s->N[u] = std::min(s->N[u], s->cases[u].max_steps);
     }
     solver_set_loops(s, st);
+    if (!e.group_on.empty() && e.set_group_mask(std::vector<uint8_t>(e.groups.size(), 1), st) && s->chunk) {
+      (void)hipGraphExecDestroy(s->chunk);
+      s->chunk = nullptr;
+    }
     launch_copy(e.W0.as<float4>(), e.Wref.as<float4>(), e.elems, st);
     s->step = 0;
     s->prepared = true;
@@ -910,6 +957,21 @@ esr_status esr_solver_finalize(esr_solver* s, void* stream) {
   });
 }
 
+namespace {
+void fill_report(const esr_solver* s, int32_t c, const CaseLoop& L, esr_case_report* out);
+}
+
+esr_status esr_solver_reports(const esr_solver* s, int32_t n, esr_case_report* out) {
+  return guard([&] {
+    if (!s || !out || n != s->eng.n_cases) throw Error(ESR_ERR_VALUE, "n must equal the number of cases");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    const Engine& e = s->eng;
+    std::vector<CaseLoop> L(e.n_cases);
+    hip_check(hipMemcpy(L.data(), e.loops_d.p, L.size() * sizeof(CaseLoop), hipMemcpyDeviceToHost), "loops D2H");
+    for (int32_t c = 0; c < n; ++c) fill_report(s, c, L[e.global_of_user[c]], out + c);
+  });
+}
+
 esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out) {
   return guard([&] {
     if (!s || !out || c < 0 || c >= s->eng.n_cases) throw Error(ESR_ERR_VALUE, "bad case index");
@@ -918,12 +980,18 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
     CaseLoop L;
     const int32_t gc = e.global_of_user[c];
     hip_check(hipMemcpy(&L, e.loops_d.as<CaseLoop>() + gc, sizeof(L), hipMemcpyDeviceToHost), "loop D2H");
+    fill_report(s, c, L, out);
+  });
+}
+
+namespace {
+void fill_report(const esr_solver* s, int32_t c, const CaseLoop& L, esr_case_report* out) {
     std::memset(out, 0, sizeof(*out));
     out->dt = s->dt[c];
-    out->n_steps = s->N[c];
+    out->n_steps = s->N[c] - L.start;
     out->converged = L.stop;
-    out->stopping_step = L.stopping_step;
-    out->steps_run = L.done ? L.steps_run : s->step;
+    out->stopping_step = L.stopping_step - L.start;
+    out->steps_run = (L.done ? L.steps_run : s->step) - L.start;
     out->last_rel = L.last_rel;
     out->final_residual = s->final_residual[c];
     // after finalize: forces of the final state; before: the last in-loop
@@ -932,14 +1000,14 @@ esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* ou
     out->cd = fin ? s->cd[c] : (L.track_eng ? L.cd : s->cd[c]);
     out->cl = fin ? s->cl[c] : (L.track_eng ? L.cl : s->cl[c]);
     out->engineering_converged = L.eng_stop;
-    out->engineering_stopping_step = L.eng_stopping_step;
+    out->engineering_stopping_step = L.eng_stopping_step - L.start;
     out->diverged = L.diverged;
     const int32_t nr = std::min(L.n_checks, RECENT_CHECKS);
     double rm = nr > 0 ? 0.0 : NAN;
     for (int32_t q = 0; q < nr; ++q) rm = std::max(rm, L.recent[q]);
     out->recent_rel_max = rm;
-  });
 }
+}  // namespace
 
 esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* group_cells, int64_t* face_items,
                            int64_t* group_faces) {
@@ -956,6 +1024,78 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
     if (group_cells) *group_cells = cells;
     if (face_items) *face_items = fitems;
     if (group_faces) *group_faces = faces;
+  });
+}
+
+esr_status esr_solver_replace_case(esr_solver* s, int32_t c, const esr_case* spec, const float* W, void* stream) {
+  return guard([&] {
+    if (!s || !spec || !W || c < 0 || c >= s->eng.n_cases) throw Error(ESR_ERR_VALUE, "bad case index");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    const esr_case& old = s->cases[c];
+    const esr_opts &a = old.opts, &b = spec->opts;
+    validate_opts(&b);
+    if (spec->mesh != old.mesh || a.flux != b.flux || a.recon != b.recon || (a.entropy != 0) != (b.entropy != 0) ||
+        a.gamma != b.gamma || a.M != b.M || a.sponge_width != b.sponge_width)
+      throw Error(ESR_ERR_VALUE, "a replacement case must use the slot's mesh and kernel configuration "
+                                 "(flux, reconstruction, entropy, gamma, M, sponge width)");
+    if (!(spec->cfl > 0.f) || !(spec->tf > 0.0)) throw Error(ESR_ERR_VALUE, "cfl and tf must be > 0");
+    hipStream_t st = S(stream);
+    if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
+    hip_check(hipStreamSynchronize(st), "caller stream sync");
+    Engine& e = s->eng;
+    const int32_t gc = e.global_of_user[c];
+    e.gather_in(c, W, e.W0.as<float4>(), st);
+    e.gather_in(c, W, e.Wref.as<float4>(), st);
+    CaseDev cd = Engine::case_params(spec->mesh, b);
+    e.cases_h[gc] = cd;
+    launch_set_case(e.cases() + gc, cd, st);
+    const float dt = e.compute_dt_one(gc, spec->cfl, st);
+    const float q = (float)spec->tf / dt;  // float32 division, as tf / dt in the reference
+    if (!(q >= 0.f) || !std::isfinite(q)) throw Error(ESR_ERR_VALUE, "non-finite time step");
+    s->cases[c] = *spec;
+    s->dt[c] = dt;
+    int64_t n_rel = (int64_t)q + 1;
+    if (spec->max_steps > 0) n_rel = std::min(n_rel, spec->max_steps);
+    s->N[c] = s->step + n_rel;
+    s->final_residual[c] = NAN;
+    s->cd[c] = NAN;
+    s->cl[c] = NAN;
+    const CaseLoop l = initial_loop(s, c, s->step);
+    hip_check(hipMemcpyAsync(e.loops_d.as<CaseLoop>() + gc, &l, sizeof(l), hipMemcpyHostToDevice, st), "loop H2D");
+    hip_check(hipStreamSynchronize(st), "replace sync");
+    // the slot's group takes part in the launches again
+    if (!e.group_on.empty() && !e.group_on[e.case_group[gc]]) {
+      std::vector<uint8_t> mask = e.group_on;
+      mask[e.case_group[gc]] = 1;
+      e.set_group_mask(mask, st);
+      if (s->chunk) {
+        (void)hipGraphExecDestroy(s->chunk);
+        s->chunk = nullptr;
+      }
+    }
+  });
+}
+
+esr_status esr_solver_compact(esr_solver* s, int32_t* groups_on, void* stream) {
+  return guard([&] {
+    if (!s) throw Error(ESR_ERR_VALUE, "null solver");
+    if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
+    Engine& e = s->eng;
+    if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
+    std::vector<CaseLoop> L(e.n_cases);
+    hip_check(hipMemcpy(L.data(), e.loops_d.p, L.size() * sizeof(CaseLoop), hipMemcpyDeviceToHost), "loops D2H");
+    std::vector<uint8_t> mask(e.groups.size(), 0);
+    for (int32_t gc = 0; gc < e.n_cases; ++gc)
+      if (!L[gc].done) mask[e.case_group[gc]] = 1;
+    if (e.set_group_mask(mask, S(stream)) && s->chunk) {
+      (void)hipGraphExecDestroy(s->chunk);
+      s->chunk = nullptr;
+    }
+    if (groups_on) {
+      int32_t n = 0;
+      for (uint8_t m : mask) n += m;
+      *groups_on = n;
+    }
   });
 }
 

@@ -143,6 +143,7 @@ struct CaseLoop {
   int32_t diverged;
   int32_t n_checks;
   double recent[16];  // check residuals, ring buffer indexed by n_checks % 16
+  int64_t start;      // solver step at which the case entered its slot (replaced cases)
 };
 constexpr int32_t RECENT_CHECKS = 16;
 

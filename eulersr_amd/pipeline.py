@@ -13,8 +13,11 @@
   (eval/live_fvm.py:37-47) -- started cold (freestream), from the IDW
   prediction, or from the network's prediction (LR solve -> features ->
   network -> HR state), with the freestream fallback for a prediction with
-  rho <= 0 or p <= 0, or a solve that diverges. All cases of a call, of any
-  geometry, run in one batched solver.
+  rho <= 0 or p <= 0, a solve that diverges, or a solve that has not reached
+  its threshold within the warm-start budget (15,000 steps: with the K = 0
+  limiter a warm start occasionally settles on a limit cycle whose residual
+  stays above the threshold set by the cold trajectory). All cases of a call,
+  of any geometry, run in one batched solver.
 """
 from __future__ import annotations
 
@@ -34,6 +37,7 @@ CHECK_EVERY, PATIENCE = 5, 3     # eval/live_fvm.py:37-47
 THRESHOLD_MARGIN = 1.001         # eval/convergence_case.py:61-66
 ENGINEERING_TOL = 0.001          # eval/live_fvm.py:47
 AUTOTUNE = True                  # solvers pick their stage path / block size (esr_solver_autotune)
+WARM_BUDGET = 15000              # steps a warm start gets before its freestream fallback
 LAUNCH = None                    # or a pinned (path, block, xcd_order) for every solver
 
 
@@ -175,6 +179,114 @@ def train(kmap, pairs, steps=3000, batch=16384, lr=1e-3, weight_decay=1e-5, seed
                      loss_last=float(np.mean(hist[-20:])), idw_mse=idw_mse)
 
 
+def _kind(sp):
+    """The solver's grouping key of a case: cases of one kind can take each
+    other's slot."""
+    kw = sp.kw
+    return (id(sp.mesh), str(kw.get("flux", kw.get("numerical_flux", ""))).lower(),
+            str(kw.get("reconstruction", kw.get("scheme", ""))).lower(), bool(kw.get("entropy", False)),
+            float(kw.get("gamma", 1.4)), float(kw.get("M", 1.0)), kw.get("sponge_width"))
+
+
+STALL = (6000, 3000, 0.97)   # (steps before the test, window, required improvement factor)
+
+
+def stalled(hist, steps, after, window, factor):
+    """True when a solve past `after` steps has not lowered its best check
+    residual below factor x the best one before the last `window` steps."""
+    if steps < after:
+        return False
+    before = [r for s, r in hist if s <= steps - window]
+    recent = [r for s, r in hist if s > steps - window]
+    return bool(before) and bool(recent) and min(recent) > factor * min(before)
+
+
+def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL):
+    """Continuous batching: solves every case of `specs` (initial states
+    `inits`) to its own stationarity criterion with at most `slots` cases in
+    flight; whenever a case is done (converged, diverged or at its step cap
+    `max_steps`), the next waiting case of the same kind (mesh and kernel
+    configuration) takes its slot (esr_solver_replace_case). Slots are split
+    over the kinds in proportion to their case counts.
+
+    fallback_inits (warm starts): a case whose solve diverges, or stalls (its
+    best check residual no longer falls, `stalled`) above its threshold, is
+    restarted in its slot from fallback_inits[i] (freestream). With the K = 0
+    limiter a warm start can settle on a limit cycle whose residual stays
+    above the threshold set by the cold trajectory.
+
+    Returns (reports in input order, seconds of solving, restarted case ids)."""
+    import torch
+    from collections import OrderedDict, deque
+    specs = [copy.copy(sp) for sp in specs]
+    if max_steps:
+        for sp in specs:
+            sp.max_steps = int(max_steps)
+    n = len(specs)
+    queues = OrderedDict()
+    for i, sp in enumerate(specs):
+        queues.setdefault(_kind(sp), deque()).append(i)
+    total = min(int(slots), n)
+    kinds = list(queues)
+    share = {k: max(1, int(round(total * len(queues[k]) / n))) for k in kinds}
+    while sum(share.values()) > total and any(v > 1 for v in share.values()):
+        k = max(share, key=lambda q: share[q])
+        share[k] -= 1
+    first = []
+    for k in kinds:
+        for _ in range(min(share[k], len(queues[k]))):
+            first.append(queues[k].popleft())
+    bs = BatchSolver([specs[i] for i in first], scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY,
+                     patience=PATIENCE, autotune=AUTOTUNE, launch=LAUNCH)
+    results = [None] * n
+    try:
+        for j, i in enumerate(first):
+            bs.set_state(j, inits[i])
+        bs.prepare()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        slot_case = list(first)
+        hist = {ci: [] for ci in first}
+        restarted = []
+        drained = False
+        step = 0
+        while any(ci is not None for ci in slot_case):
+            step = bs.run(step + chunk)
+            reps = bs.reports()
+            for j, ci in enumerate(slot_case):
+                if ci is None:
+                    continue
+                r = reps[j]
+                done = r["converged"] or r["diverged"] or r["steps_run"] >= r["N"]
+                if fallback_inits is not None and ci not in restarted and not r["converged"]:
+                    if r["last_rel"] == r["last_rel"] and r["last_rel"] != float("inf"):
+                        hist[ci].append((r["steps_run"], r["last_rel"]))
+                    if r["diverged"] or (not done and stalled(hist[ci], r["steps_run"], *stall)):
+                        bs.replace_case(j, specs[ci], fallback_inits[ci])
+                        restarted.append(ci)
+                        continue
+                if not done:
+                    continue
+                results[ci] = r
+                q = queues[_kind(specs[ci])]
+                if q:
+                    nxt = q.popleft()
+                    bs.replace_case(j, specs[nxt], inits[nxt])
+                    slot_case[j] = nxt
+                    hist[nxt] = []
+                else:
+                    slot_case[j] = None
+                    drained = True
+            if drained:
+                bs.compact()        # groups with no case left stop launching
+                drained = False
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+    finally:
+        bs.close()
+    return results, t, restarted
+
+
 @dataclass
 class WarmCase:
     """One (geometry, Mach, alpha) case of the warm-start harness: its LR
@@ -221,10 +333,60 @@ class WarmStarter:
         torch.cuda.synchronize()
         return W.reshape(len(grid), kmap.n_hr, 4), bad.cpu().numpy().astype(bool)
 
-    def run(self, cases, mode="mlp", max_steps=0):
+    def initial_states(self, cases, mode="mlp", max_steps=0):
+        """HR initial states of `cases` for a start mode: freestream ("cold"),
+        or LR fixed-time solves (one batch) -> features -> IDW ("idw") or
+        network ("mlp"), with the freestream fallback for non-physical
+        predictions. Returns (states, fallback flags, LR seconds, inference
+        seconds)."""
+        import torch
+        n = len(cases)
+        inits = [c.hr_init for c in cases]
+        fallback = [False] * n
+        t_lr = t_inf = 0.0
+        if mode != "cold":
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps)
+            torch.cuda.synchronize()
+            t_lr = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            for g in dict.fromkeys(c.geometry for c in cases):
+                ids = [i for i, c in enumerate(cases) if c.geometry == g]
+                W, bad = self.predict(g, torch.stack([lr_states[i] for i in ids]),
+                                      [(cases[i].mach, cases[i].aoa) for i in ids], use_net=(mode == "mlp"))
+                for j, i in enumerate(ids):
+                    fallback[i] = bool(bad[j])
+                    if not bad[j]:
+                        inits[i] = W[j]
+            t_inf = time.perf_counter() - t0
+        return inits, fallback, t_lr, t_inf
+
+    def campaign(self, cases, mode="mlp", slots=128, max_steps=0, lr_max_steps=0):
+        """The warm-start protocol over a campaign of cases with continuous
+        batching (`campaign`): LR solves and predictions for every case, then
+        the HR solves, `slots` at a time. A warm-started case that diverges
+        is solved again from freestream. Returns a summary like `run`."""
+        import torch
+        torch.cuda.synchronize()
+        t_all = time.perf_counter()
+        inits, fallback, t_lr, t_inf = self.initial_states(cases, mode, lr_max_steps)
+        fb = None if mode == "cold" else [c.hr_init for c in cases]
+        reps, t, restarted = campaign([c.hr_spec for c in cases], inits, slots=slots, max_steps=max_steps,
+                                      fallback_inits=fb)
+        for i in restarted:
+            fallback[i] = True
+        torch.cuda.synchronize()
+        return dict(mode=mode, lr_s=t_lr, infer_s=t_inf, hr_s=t, total_s=time.perf_counter() - t_all,
+                    fallback=fallback, retried=restarted, reports=reps)
+
+    def run(self, cases, mode="mlp", max_steps=0, warm_budget=WARM_BUDGET):
         """One batched pass. mode: "cold" (freestream), "idw" or "mlp".
-        Returns per-case reports and the wall-time split (LR solve, inference,
-        HR solve, total)."""
+        A warm-started case falls back to freestream (a second batch of such
+        cases, up to max_steps) when its prediction is non-physical, its solve
+        diverges, or it has not reached its threshold within `warm_budget`
+        steps (0: no budget). Returns per-case reports and the wall-time split
+        (LR solve, inference, HR solve, total)."""
         import torch
         n = len(cases)
         out = dict(mode=mode, lr_s=0.0, infer_s=0.0, hr_s=0.0, fallback=[False] * n, retried=[])
@@ -246,10 +408,15 @@ class WarmStarter:
                     if not bad[j]:
                         inits[i] = W[j]
             out["infer_s"] = time.perf_counter() - t0
-        reps, states, t = converge([c.hr_spec for c in cases], inits, max_steps=max_steps)
+        warm_cap = max_steps
+        if mode != "cold" and warm_budget:
+            warm_cap = min(max_steps, warm_budget) if max_steps else warm_budget
+        reps, states, t = converge([c.hr_spec for c in cases], inits, max_steps=warm_cap)
         out["hr_s"] = t
-        # a warm start whose solve diverges is re-run from freestream
-        div = [i for i, r in enumerate(reps) if r["diverged"] and mode != "cold" and not out["fallback"][i]]
+        # a warm start whose solve diverges, or that has not converged within
+        # the budget, is re-run from freestream
+        div = [i for i, r in enumerate(reps)
+               if mode != "cold" and not out["fallback"][i] and (r["diverged"] or not r["converged"])]
         if div:
             r2, s2, t2 = converge([cases[i].hr_spec for i in div], [cases[i].hr_init for i in div],
                                   max_steps=max_steps)

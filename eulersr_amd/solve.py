@@ -29,6 +29,7 @@ class CaseSpec:
     engineering: tuple | None = None  # (target_cd, target_cl, cd_tol_abs, cl_tol_abs, patience)
     threshold: float = 0.0         # > 0 overrides the solver's stationarity threshold
     tag: dict = field(default_factory=dict)
+    max_steps: int = 0             # > 0: per-case step cap (N = min(N, max_steps))
 
 
 def transonic_cfl(mach: float, cfl: float) -> float:
@@ -61,18 +62,7 @@ class BatchSolver:
         for i, cs in enumerate(self.specs):
             dm = device_mesh(cs.mesh, cs.kw)
             self._meshes.append(dm)
-            arr[i].mesh = dm.handle
-            arr[i].opts = make_opts(cs.kw)
-            arr[i].cfl = float(cs.cfl)
-            arr[i].tf = float(cs.tf)
-            arr[i].q_ref = float(cs.q_ref)
-            arr[i].threshold = float(cs.threshold)
-            if cs.engineering is not None:
-                tcd, tcl, cdt, clt, pat = cs.engineering
-                arr[i].track_engineering = 1
-                arr[i].target_cd, arr[i].target_cl = float(tcd), float(tcl)
-                arr[i].cd_tol_abs, arr[i].cl_tol_abs = float(cdt), float(clt)
-                arr[i].engineering_patience = int(pat)
+            self._fill_case(arr[i], cs, dm)
         so = L.SolveOpts()
         so.scheme = SCHEME[str(scheme).upper()]
         so.threshold = float(threshold)
@@ -86,6 +76,44 @@ class BatchSolver:
         self.handle = h
         self.n = n
         self.steps = 0
+
+    @staticmethod
+    def _fill_case(c, cs, dm):
+        c.mesh = dm.handle
+        c.opts = make_opts(cs.kw)
+        c.cfl = float(cs.cfl)
+        c.tf = float(cs.tf)
+        c.q_ref = float(cs.q_ref)
+        c.threshold = float(cs.threshold)
+        c.max_steps = int(cs.max_steps)
+        if cs.engineering is not None:
+            tcd, tcl, cdt, clt, pat = cs.engineering
+            c.track_engineering = 1
+            c.target_cd, c.target_cl = float(tcd), float(tcl)
+            c.cd_tol_abs, c.cl_tol_abs = float(cdt), float(clt)
+            c.engineering_patience = int(pat)
+
+    def replace_case(self, i, spec, W):
+        """Continuous batching: case `spec` (same mesh and kernel
+        configuration as slot i) enters slot i with initial state W; its
+        steps count from the current solver step."""
+        import torch
+        t, _ = as_device_state(W)
+        dm = device_mesh(spec.mesh, spec.kw)
+        c = L.Case()
+        self._fill_case(c, spec, dm)
+        L.check(L.lib().esr_solver_replace_case(self.handle, int(i), C.byref(c), C.c_void_p(t.data_ptr()),
+                                                stream_ptr()), "esr_solver_replace_case")
+        torch.cuda.current_stream().synchronize()
+        self.specs[i] = spec
+        self._meshes[i] = dm
+
+    def compact(self):
+        """Drops groups whose cases are all done from the launches; returns
+        the number of groups still launched."""
+        n = C.c_int32()
+        L.check(L.lib().esr_solver_compact(self.handle, C.byref(n), stream_ptr()), "esr_solver_compact")
+        return n.value
 
     def set_state(self, i, W):
         t, _ = as_device_state(W)
@@ -143,6 +171,16 @@ class BatchSolver:
     def report(self, i) -> dict:
         r = L.CaseReport()
         L.check(L.lib().esr_solver_report(self.handle, i, C.byref(r)), "esr_solver_report")
+        return self._report_dict(r)
+
+    def reports(self) -> list:
+        """Reports of every case (one device copy)."""
+        arr = (L.CaseReport * self.n)()
+        L.check(L.lib().esr_solver_reports(self.handle, self.n, arr), "esr_solver_reports")
+        return [self._report_dict(r) for r in arr]
+
+    @staticmethod
+    def _report_dict(r) -> dict:
         return dict(dt=np.float32(r.dt), N=int(r.n_steps), converged=bool(r.converged),
                     stopping_step=int(r.stopping_step), steps_run=int(r.steps_run),
                     last_rel=float(r.last_rel), final_residual=float(r.final_residual),

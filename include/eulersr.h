@@ -143,6 +143,9 @@ typedef struct {
   double threshold;          /* > 0: per-case stationarity threshold (the
                                 HR-probe x 1.001 rule of eval/convergence_case.py:61-66);
                                 <= 0: esr_solve_opts.threshold */
+  int64_t max_steps;         /* > 0: N = min(int(tf/dt) + 1, max_steps), a per-case
+                                step cap of the campaign harness (no reference
+                                counterpart); 0: none */
 } esr_case;
 
 typedef struct {
@@ -184,9 +187,26 @@ esr_status esr_solver_prepare(esr_solver* s, void* stream);
  * absolute step `until_step` when > 0); *steps = steps taken since prepare.
  * Successive calls continue the same loop. */
 esr_status esr_solver_run(esr_solver* s, int64_t until_step, int64_t* steps, void* stream);
+/* Continuous batching: puts a new case into slot c of a prepared solver,
+ * between two esr_solver_run calls, typically once the slot's case is done.
+ * The new case must use the slot's mesh and kernel configuration (flux,
+ * reconstruction, entropy, gamma, M, sponge width); its parameters, dt
+ * (from W, device pointer, original cell order), N, stationarity state and
+ * reference state are set afresh, and its steps count from the current
+ * solver step (reports give steps relative to that start). The other cases
+ * are untouched. No counterpart in the reference (one case per process). */
+esr_status esr_solver_replace_case(esr_solver* s, int32_t c, const esr_case* spec, const float* W, void* stream);
+/* Continuous batching: drops from the launches every group (cases sharing a
+ * mesh and kernel configuration) whose cases are all done, until a case
+ * enters one of its slots again (esr_solver_replace_case). *groups_on (may be
+ * null) receives the number of groups still launched. No counterpart in the
+ * reference. */
+esr_status esr_solver_compact(esr_solver* s, int32_t* groups_on, void* stream);
 /* One-step probe residual + CD/CL of the current states. */
 esr_status esr_solver_finalize(esr_solver* s, void* stream);
 esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out);
+/* Reports of all n cases (n = the solver's case count) with one device copy. */
+esr_status esr_solver_reports(const esr_solver* s, int32_t n, esr_case_report* out);
 
 /* -- learned warm start ----------------------------------------------------- */
 /* The reference's warm start is a Flax network over kNN/IDW features

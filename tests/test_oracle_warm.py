@@ -222,3 +222,13 @@ def test_features_oracle_columns():
     d[n_hr + 2, 3] = -10.0
     _, bad = OW.apply(idw, d, sig, g, n_hr)
     assert list(bad) == [False, True]
+
+
+def test_stall_rule():
+    from eulersr_amd.pipeline import stalled
+    falling = [(s, 1e-4 * 0.9995 ** s) for s in range(100, 12000, 100)]
+    flat = [(s, 2e-5 * (1.0 + 0.1 * ((s // 100) % 2))) for s in range(100, 12000, 100)]
+    assert not stalled(falling, 11900, 6000, 3000, 0.97)
+    assert stalled(flat, 11900, 6000, 3000, 0.97)
+    assert not stalled(flat, 5000, 6000, 3000, 0.97)       # too early
+    assert not stalled([], 11900, 6000, 3000, 0.97)
