@@ -29,11 +29,12 @@ with the next waiting case of its kind as soon as its case is done
 timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
 the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
 the warm start against those plain solves, the same campaign from cold
-starts, the stage kernels' HBM rates at a 256-case batch, and the
+starts, the stage kernels' HBM rates at the campaign's 64-case launch, and the
 reference's single-case configuration (6 cases: diamond + NACA0012, M in
 {0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
-for the slowest stage kernel of the autotuned launch at the 256-case batch,
-timed with HIP events on the solver's stream over back-to-back launches; its
+for the slowest stage kernel of the autotuned launch of the campaign's
+64-case batch, timed with HIP events on the solver's stream over
+back-to-back launches; its
 algorithmic bytes per launch come from the launch's work-items
 (``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
 
@@ -103,11 +104,14 @@ def traffic_file(tag):
     return os.path.join(ROOT, "profiles", f"traffic_bench_{tag}.txt")
 
 
-def measured_traffic(kernel, tuning, tag):
-    """HBM bytes per launch of `kernel` (summed over the flux classes of the
-    launch set) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    of this workload (calibrated against a 1 GiB copy), when that summary was
-    taken with the same launch configuration; else None."""
+def measured_traffic(kernel, tuning, tag, n_classes=1):
+    """HBM bytes per launch of `kernel` over the whole launch set (one
+    dispatch per kernel class, n_classes of them) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (per-dispatch
+    means, calibrated against a 1 GiB copy), when that summary was taken with
+    the same launch configuration; else None. Kernels templated on the flux
+    (k_face, k_cell) have one instance per class; k_recon and k_update<2>
+    are one instance dispatched once per class."""
     path = traffic_file(tag)
     if not os.path.exists(path):
         return None
@@ -118,7 +122,9 @@ def measured_traffic(kernel, tuning, tag):
     pre = PROFILE_PREFIX[kernel]
     hits = [fb + wb for name, (fb, wb) in read_traffic(path).items()
             if (name.startswith(pre) if pre.endswith("<") else name == pre)]
-    return sum(hits) if hits else None
+    if not hits:
+        return None
+    return sum(hits) if pre.endswith("<") else hits[0] * n_classes
 
 
 def kernel_table(sizes, ms):
@@ -416,8 +422,10 @@ def main():
         log(f"cold-start campaign: {cold['total_s']:.2f}s -> {rate(cold):.0f} cases/hour, steps median "
             f"{np.median(cst) if cst else 0:.0f} mean {np.mean(cst) if cst else 0:.0f}")
 
-    kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, P.LAUNCH)
-    log(f"autotuned launch of the batch: {tuning}")
+    # the stage kernels at the campaign's launch: its first `slots` cases
+    _, first = P.initial_slots([c.hr_spec for c in cases], args.slots)
+    kern, sizes, tuning = stage_kernels([cases[i] for i in first], args.kernel_reps, P.LAUNCH)
+    log(f"autotuned launch of the {len(first)}-slot batch: {tuning}")
     for name, k in kern.items():
         log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.1f} MB -> "
             f"{k['gbs']:.0f} GB/s")
@@ -443,7 +451,8 @@ def main():
                    "cases": args.cases, "h": args.h, "h_lr": args.h_lr, "cells_total": int(sum(cells))},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, tuning, f"{args.cases}case_h{args.h}"),
+                     "traffic": measured_traffic(dom, tuning, f"{args.slots}slot_h{args.h}",
+                                                 len({c.hr_spec.kw["flux"] for c in cases})),
                      "kernel": dom, "stage_path": tuning["path"], "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,

@@ -201,6 +201,27 @@ def stalled(hist, steps, after, window, factor):
     return bool(before) and bool(recent) and min(recent) > factor * min(before)
 
 
+def initial_slots(specs, slots):
+    """Per-kind queues of case ids and the cases that take the `slots` slots
+    first: slots split over the kinds in proportion to their case counts (at
+    least one each), cases in input order."""
+    from collections import OrderedDict, deque
+    n = len(specs)
+    queues = OrderedDict()
+    for i, sp in enumerate(specs):
+        queues.setdefault(_kind(sp), deque()).append(i)
+    total = min(int(slots), n)
+    share = {k: max(1, int(round(total * len(queues[k]) / n))) for k in queues}
+    while sum(share.values()) > total and any(v > 1 for v in share.values()):
+        k = max(share, key=lambda q: share[q])
+        share[k] -= 1
+    first = []
+    for k in queues:
+        for _ in range(min(share[k], len(queues[k]))):
+            first.append(queues[k].popleft())
+    return queues, first
+
+
 def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL):
     """Continuous batching: solves every case of `specs` (initial states
     `inits`) to its own stationarity criterion with at most `slots` cases in
@@ -217,25 +238,12 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
 
     Returns (reports in input order, seconds of solving, restarted case ids)."""
     import torch
-    from collections import OrderedDict, deque
     specs = [copy.copy(sp) for sp in specs]
     if max_steps:
         for sp in specs:
             sp.max_steps = int(max_steps)
     n = len(specs)
-    queues = OrderedDict()
-    for i, sp in enumerate(specs):
-        queues.setdefault(_kind(sp), deque()).append(i)
-    total = min(int(slots), n)
-    kinds = list(queues)
-    share = {k: max(1, int(round(total * len(queues[k]) / n))) for k in kinds}
-    while sum(share.values()) > total and any(v > 1 for v in share.values()):
-        k = max(share, key=lambda q: share[q])
-        share[k] -= 1
-    first = []
-    for k in kinds:
-        for _ in range(min(share[k], len(queues[k]))):
-            first.append(queues[k].popleft())
+    queues, first = initial_slots(specs, slots)
     bs = BatchSolver([specs[i] for i in first], scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY,
                      patience=PATIENCE, autotune=AUTOTUNE, launch=LAUNCH)
     results = [None] * n
