@@ -324,7 +324,7 @@ __device__ __forceinline__ float4 entropy_vars(const float4 W, const Phys& ph) {
   const float r = W.x, ir = frcp(r), u = W.y * ir, v = W.z * ir;
   const float P = (W.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
   const float iP = frcp(P);
-  const float s = logf(P / powf(r, ph.g));
+  const float s = logf(P) - ph.g * logf(r);  // ln(P / rho^gamma)
   float4 V;
   V.x = (ph.g - s) * ph.inv_gm1 - 0.5f * r * (u * u + v * v) * iP;
   V.y = W.y * iP;
@@ -455,9 +455,23 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float
   const float lam[4] = {un - a, un, un, un + a};
   const float ig = frcp(ph.g), i2g = frcp(ph.two_g);
   const float S[4] = {r * i2g, ph.gm1 * r * ig, r * ig, r * i2g};
-  const float4 eR = entropy_vars(WR, ph);
-  const float4 eL = entropy_vars(WL, ph);
-  const float dv[4] = {eR.x - eL.x, eR.y - eL.y, eR.z - eL.z, eR.w - eL.w};
+  // difference of the entropy variables V(W_R) - V(W_L) (helper.py:58-69)
+  // from the unclamped states: V = [(g - s)/(g - 1) - rho |u|^2 / (2P),
+  // rho u / P, rho v / P, -rho / P], s = ln(P / rho^g); the entropy jump
+  // taken as ln(P_R/P_L) - g ln(rho_R/rho_L) (two logarithms, no power)
+  float dv[4];
+  {
+    const float r0 = WL.x, r1 = WR.x;
+    const float i0 = frcp(r0), i1 = frcp(r1);
+    const float q0 = WL.y * WL.y + WL.z * WL.z, q1 = WR.y * WR.y + WR.z * WR.z;  // rho^2 |u|^2
+    const float P0 = (WL.w - 0.5f * q0 * i0) * ph.gm1, P1 = (WR.w - 0.5f * q1 * i1) * ph.gm1;
+    const float iP0 = frcp(P0), iP1 = frcp(P1);
+    const float ds = logf(P1 * iP0) - ph.g * logf(r1 * i0);
+    dv[0] = -ds * ph.inv_gm1 - 0.5f * (q1 * i1 * iP1 - q0 * i0 * iP0);
+    dv[1] = WR.y * iP1 - WL.y * iP0;
+    dv[2] = WR.z * iP1 - WL.z * iP0;
+    dv[3] = r0 * iP0 - r1 * iP1;
+  }
   float DRH[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
