@@ -224,6 +224,19 @@ struct Engine {
       GroupDev g{};
       g.m = m->dev;
       g.C = (int32_t)members.size();
+      // magic number of the unsigned division by C (round-up method)
+      if (g.C > 1) {
+        int32_t l = 0;
+        while ((1u << l) < (uint32_t)g.C) ++l;
+        g.cshift = l;
+        g.cmagic = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - (uint64_t)g.C)) / (uint64_t)g.C + 1);
+      } else {
+        g.cshift = 0;
+        g.cmagic = 0;
+      }
+      // in-group indices are 32-bit: (3 n + nf) C work-items must stay below 2^31
+      if ((3 * (int64_t)m->n + (int64_t)m->dev.nf) * g.C >= ((int64_t)1 << 31))
+        throw Error(ESR_ERR_VALUE, "too many cases on one mesh with one kernel configuration for one solver");
       g.case0 = case0;
       g.st_off = st;
       g.ph = make_phys(std::get<4>(k), std::get<5>(k));

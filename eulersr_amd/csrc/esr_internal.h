@@ -94,6 +94,8 @@ struct Phys {
 struct GroupDev {
   MeshDev m;
   int32_t C;
+  uint32_t cmagic;       // t / C = (h + ((t - h) >> 1)) >> (cshift - 1), h = umulhi(cmagic, t); C > 1
+  int32_t cshift;
   int32_t case0;         // global index of the group's first case
   int64_t st_off;        // float4 offset of the group's state slab
   int64_t rec_off;       // float4 offset of the group's reconstruction slab ([3][n][C])

@@ -120,26 +120,33 @@ __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, co
   if (gi < 0) return false;
   const GroupDev* G = groups + gi;
   const int32_t C = G->C;
-  const int64_t t = (int64_t)(lb - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
-  const int32_t idx = (int32_t)(t / C);
-  const int32_t c = (int32_t)(t - (int64_t)idx * C);
+  // t / C by the group's magic number (in-group indices are 32-bit)
+  const uint32_t t = (uint32_t)(lb - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
+  uint32_t q = t;
+  if (C > 1) {
+    const uint32_t h = __umulhi(G->cmagic, t);
+    q = (h + ((t - h) >> 1)) >> (G->cshift - 1);
+  }
+  const int32_t idx = (int32_t)q;
+  const int32_t c = (int32_t)(t - q * (uint32_t)C);
   if (idx >= (FACES ? G->m.nf : G->m.n)) return false;
   if (!cases[G->case0 + c].active) return false;
   it.G = G;
   it.idx = idx;
   it.c = c;
-  it.sidx = FACES ? 0 : G->st_off + (int64_t)idx * C + c;
+  it.sidx = FACES ? 0 : G->st_off + (int64_t)(int32_t)t;
   return true;
 }
 
+// element offsets inside a group are 32-bit (checked when the groups are built)
 __device__ __forceinline__ int64_t sidx_of(const GroupDev* G, int32_t cell, int32_t c) {
-  return G->st_off + (int64_t)cell * G->C + c;
+  return G->st_off + (int64_t)(cell * G->C + c);
 }
 __device__ __forceinline__ int64_t ridx_of(const GroupDev* G, int k, int32_t cell, int32_t c) {
-  return G->rec_off + ((int64_t)k * G->m.n + cell) * G->C + c;
+  return G->rec_off + (int64_t)((k * G->m.n + cell) * G->C + c);
 }
 __device__ __forceinline__ int64_t fidx_of(const GroupDev* G, int32_t face, int32_t c) {
-  return G->fx_off + (int64_t)face * G->C + c;
+  return G->fx_off + (int64_t)(face * G->C + c);
 }
 
 // ---------------------------------------------------------------------------
@@ -170,11 +177,11 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
   bool bnd[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[(int64_t)k * n + cell];
+    const uint32_t f = m.fref[k * n + cell];
     float4 WR;
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? m.nrm[(int64_t)k * n + cell] : make_float2(0.f, 0.f);
+      const float2 nk = (kind == G_WALL) ? m.nrm[k * n + cell] : make_float2(0.f, 0.f);
       WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
       bnd[k] = true;
     } else {
@@ -189,7 +196,7 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
   const float4 ai = m.ainv[cell];
   float2 dm[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) dm[k] = m.dxm[(int64_t)k * n + cell];
+  for (int k = 0; k < 3; ++k) dm[k] = m.dxm[k * n + cell];
 
   const float qi[4] = {pi.r, pi.u, pi.v, pi.P};
   float D[3][4];
@@ -212,10 +219,10 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
     for (int k = 0; k < 3; ++k) {
       const float d = dm[k].x * gx + dm[k].y * gy;
       D[k][v] = d;
-      float p = 1.0f;
-      if (d > 1e-7f) p = venkat(mx - qi[v], d);
-      if (d < -1e-7f) p = venkat(mn - qi[v], d);
-      p3[k] = p;
+      // phi = venkat(max - q, d) for d > 1e-7, venkat(min - q, d) for
+      // d < -1e-7, else 1 (branch-free: one evaluation per face)
+      const float pv = venkat((d > 0.f ? mx : mn) - qi[v], d);
+      p3[k] = (fabsf(d) > 1e-7f) ? pv : 1.0f;
     }
     phi[v] = fminf(fminf(p3[0], p3[1]), p3[2]);
   }
@@ -724,7 +731,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t cf = m.cface[(int64_t)k * n + cell];
+    const uint32_t cf = m.cface[k * n + cell];
     const float4 f = fx[fidx_of(G, (int32_t)(cf & ~CFACE_NEG), it.c)];
     Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
@@ -756,9 +763,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[(int64_t)k * n + cell];
-    const float2 nk = m.nrm[(int64_t)k * n + cell];
-    const float sk = m.len[(int64_t)k * n + cell];
+    const uint32_t f = m.fref[k * n + cell];
+    const float2 nk = m.nrm[k * n + cell];
+    const float sk = m.len[k * n + cell];
     float4 WL, WR;
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
@@ -905,9 +912,9 @@ __global__ __launch_bounds__(BLOCK) void k_dt(const GroupDev* __restrict__ gdev,
       float tk[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const float2 nk = m.nrm[(int64_t)k * m.n + cell];
+        const float2 nk = m.nrm[k * m.n + cell];
         const float un = q.u * nk.x + q.v * nk.y;
-        tk[k] = (cel + fabsf(un)) * m.len[(int64_t)k * m.n + cell];
+        tk[k] = (cel + fabsf(un)) * m.len[k * m.n + cell];
       }
       const float d = m.area[cell] / ((tk[0] + tk[1]) + tk[2]);
       acc = (d != d) ? d : ((acc != acc) ? acc : fminf(acc, d));
