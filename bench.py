@@ -2,7 +2,7 @@
 against the plain single-case GPU solver.
 
 Workload (one GPU) -- the full form of the warm-start metric: a ragged batch
-of 256 HR cases at h = 0.025 over five geometries (a diamond with half-angle
+of 1,024 HR cases at h = 0.025 over five geometries (a diamond with half-angle
 5 deg, NACA0012, NACA2412, NACA4412, NACA0015; 22k-34k cells each), with
 (M, alpha) drawn by default_rng(0) from M in [0.7, 3.0] (step 0.01) x alpha
 in [-5, 5] (step 0.5), half HLLC and half Roe flux, MUSCL reconstruction,
@@ -23,17 +23,17 @@ a test case) and the training of one warm-start network per geometry.
 A "step" of this benchmark is one warm-start campaign over the 256 cases:
 LR fixed-time solves of all cases in one solver, kNN/IDW features and MLP
 inference on the GPU, HR states with the freestream fallback, then the HR
-solves with continuous batching -- 64 cases in flight, each slot refilled
+solves with continuous batching -- 128 cases in flight, each slot refilled
 with the next waiting case of its kind as soon as its case is done
 (esr_solver_replace_case). ``value`` = converged cases per hour over the K
 timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
 the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
 the warm start against those plain solves, the same campaign from cold
-starts, the stage kernels' HBM rates at the campaign's 64-case launch, and the
+starts, the stage kernels' HBM rates at the campaign's 128-case launch, and the
 reference's single-case configuration (6 cases: diamond + NACA0012, M in
 {0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
 for the slowest stage kernel of the autotuned launch of the campaign's
-64-case batch, timed with HIP events on the solver's stream over
+128-case batch, timed with HIP events on the solver's stream over
 back-to-back launches; its
 algorithmic bytes per launch come from the launch's work-items
 (``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
@@ -239,27 +239,46 @@ def cd_drift(cases, stops, factor=2):
 
 
 def cpu_baseline(mesh, spec, steps_per_case, target_s=12.0):
-    """numpy oracle (float32) SSP-RK2 steps on one HR mesh, 1 process."""
+    """The CPU restatement timed on this host: oracle/cfvm.c (float32,
+    OpenMP over cells, bit-identical to the numpy oracle) running SSP-RK2
+    HLLC + MUSCL steps on one HR mesh with every core this process may use,
+    for about target_s seconds; converged cases/hour extrapolated with the
+    GPU-measured mean steps of a plain converged case. The numpy oracle in
+    float64 (one process) is timed beside it for a few seconds."""
+    from oracle import cfvm as CF
     from oracle import fvm as F
+    threads = len(os.sched_getaffinity(0))
+    CF.set_threads(threads)
     om = F.omesh(mesh, np.float32)
+    cm = CF.CMesh(om)
     W = np.asarray(spec.kw["value"], dtype=np.float32)[None, :].repeat(mesh.n_cells, 0)
-    kw = dict(spec.kw)
-    dt = F.get_dt(W, om, CFL=spec.cfl)
+    kw = dict(spec.kw, flux="HLLC", reconstruction="MUSCL")
+    dt = CF.get_dt(W, cm, CFL=spec.cfl)
     t0 = time.perf_counter()
-    W = F.time_step_RK2_SSP(W, om, dt, **kw)
-    one = time.perf_counter() - t0
-    n = max(1, int(target_s / max(one, 1e-3)))
+    W = CF.srk2_steps(W, cm, dt, 5, **kw)
+    one = (time.perf_counter() - t0) / 5
+    n = max(5, int(target_s / max(one, 1e-4)))
     t0 = time.perf_counter()
-    for _ in range(n):
-        W = F.time_step_RK2_SSP(W, om, dt, **kw)
+    W = CF.srk2_steps(W, cm, dt, n, **kw)
     t = time.perf_counter() - t0
     cups = n * mesh.n_cells / t
     cases_h = cups / (mesh.n_cells * steps_per_case) * 3600.0
-    return dict(value=cases_h, unit="cases/hour", cores=1, kind="port",
-                sample=(f"numpy oracle float32, {n} SSP-RK2 steps on diamond h=0.025 ({mesh.n_cells} cells) "
-                        f"in {t:.1f}s = {cups:.3g} cell-updates/s; cases/hour extrapolated with the GPU-measured "
-                        f"mean {steps_per_case:.0f} steps per converged plain case"),
-                cell_updates_per_s=cups)
+    # numpy oracle, float64, one process
+    om64 = F.omesh(mesh, np.float64)
+    W64 = W.astype(np.float64)
+    kw64 = dict(kw, value=np.asarray(spec.kw["value"], dtype=np.float64))
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < 3.0:
+        W64 = F.time_step_RK2_SSP(W64, om64, float(dt), **kw64)
+        k += 1
+    cups64 = k * mesh.n_cells / (time.perf_counter() - t0)
+    return dict(value=cases_h, unit="cases/hour", cores=threads, kind="port",
+                sample=(f"oracle/cfvm.c (C restatement, float32, OpenMP {threads} threads), {n} SSP-RK2 HLLC+MUSCL "
+                        f"steps on diamond h=0.025 ({mesh.n_cells} cells) in {t:.1f}s = {cups:.3g} cell-updates/s; "
+                        f"cases/hour extrapolated with the GPU-measured mean {steps_per_case:.0f} steps per "
+                        f"converged plain case"),
+                cell_updates_per_s=cups, numpy_float64_cell_updates_per_s=cups64)
 
 
 def rate(run):
@@ -308,8 +327,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--cases", type=int, default=256, help="cases of the ragged warm-start batch")
-    ap.add_argument("--slots", type=int, default=64, help="cases in flight (continuous batching)")
+    ap.add_argument("--cases", type=int, default=1024, help="cases of the ragged warm-start batch")
+    ap.add_argument("--slots", type=int, default=128, help="cases in flight (continuous batching)")
     ap.add_argument("--single-sample", type=int, default=16,
                     help="cases of the batch run alone by the plain single-case solver")
     ap.add_argument("--h", type=float, default=0.025)

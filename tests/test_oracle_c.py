@@ -1,0 +1,124 @@
+"""The C restatement of the CPU oracle (oracle/cfvm.c, the timed CPU
+baseline) against the numpy oracle (oracle/fvm.py): bit-identical float32
+residuals, dt and SSP-RK2 steps for HLLC with MUSCL and constant
+reconstruction, over every boundary kind (slip walls of two markers, inflow,
+outflow, subsonic inlet, an unknown marker that wraps to the last cell),
+non-default gamma/M scaling and sponge parameters, at any thread count."""
+import copy
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import base_cfg, mesh_of, perturbed_state
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def CF():
+    if not os.path.exists(os.path.join(ROOT, "oracle", "libcfvm.so")):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    from oracle import cfvm
+    return cfvm
+
+
+def _bits_equal(a, b):
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def _remarked(om):
+    """A copy of the mesh view whose outlet faces take, in turn, markers 5
+    (subsonic inlet), 6 (a second wall marker) and 7 (unknown)."""
+    om = copy.copy(om)
+    fm = om.face_markers.copy()
+    out = np.nonzero(fm == 4)[0]
+    fm[out[0::4]] = 5
+    fm[out[1::4]] = 6
+    fm[out[2::4]] = 7
+    om.face_markers = fm
+    return om
+
+
+CASES = [
+    ("diamond", 0.2, dict(Mach=2.5, aoa=5.0), {}),
+    ("naca0012", 0.2, dict(Mach=0.8, aoa=1.25, case="naca0012"), {}),
+    ("diamond", 0.2, dict(Mach=1.3, aoa=-3.0), dict(gamma=1.3, M=2.0)),
+    ("diamond", 0.2, dict(Mach=2.0, aoa=0.0), dict(sponge_width=1.5, sponge_strength=0.7)),
+]
+
+
+@pytest.mark.parametrize("geom,h,over,extra", CASES)
+@pytest.mark.parametrize("recon", ["MUSCL", "constant"])
+@pytest.mark.parametrize("remark", [False, True])
+def test_residual_bit_identical(CF, geom, h, over, extra, recon, remark):
+    from oracle import fvm as F
+    mesh = mesh_of(geom, h)
+    cfg = base_cfg(**over)
+    W, inlet = perturbed_state(mesh, cfg)
+    om = F.omesh(mesh, np.float32)
+    wm = None
+    if remark:
+        om = _remarked(om)
+        wm = [2, 6]
+    kw = dict(gamma=1.4, M=1.0, reconstruction=recon, flux="HLLC", value=inlet, entropy=False)
+    kw.update(extra)
+    if wm:
+        kw["wall_markers"] = wm
+    R = F.residual(W, om, **kw)
+    Rc = CF.residual(W, CF.CMesh(om, wm), **kw)
+    assert np.all(np.isfinite(R))
+    assert _bits_equal(R, Rc), float(np.max(np.abs(R - Rc)))
+
+
+@pytest.mark.parametrize("geom,h,over", [("diamond", 0.2, dict(Mach=2.5, aoa=5.0)),
+                                         ("naca0012", 0.2, dict(Mach=0.8, aoa=1.25, case="naca0012"))])
+def test_dt_and_steps_bit_identical(CF, geom, h, over):
+    from oracle import fvm as F
+    mesh = mesh_of(geom, h)
+    cfg = base_cfg(**over)
+    W, inlet = perturbed_state(mesh, cfg)
+    om = F.omesh(mesh, np.float32)
+    cm = CF.CMesh(om)
+    for cfl, g in ((0.6, 1.4), (0.45, 1.3)):
+        assert _bits_equal(F.get_dt(W, om, CFL=cfl, gamma=g), CF.get_dt(W, cm, CFL=cfl, gamma=g))
+    kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=inlet, entropy=False)
+    dt = F.get_dt(W, om, CFL=0.6)
+    Wn = W
+    for _ in range(3):
+        Wn = F.time_step_RK2_SSP(Wn, om, dt, **kw)
+    assert _bits_equal(CF.time_step_RK2_SSP(W, cm, dt, **kw), F.time_step_RK2_SSP(W, om, dt, **kw))
+    assert _bits_equal(CF.srk2_steps(W, cm, dt, 3, **kw), Wn)
+
+
+def test_thread_count_invariant(CF):
+    from oracle import fvm as F
+    mesh = mesh_of("naca0012", 0.2)
+    cfg = base_cfg(Mach=1.2, aoa=2.0, case="naca0012")
+    W, inlet = perturbed_state(mesh, cfg)
+    cm = CF.CMesh(F.omesh(mesh, np.float32))
+    kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=inlet, entropy=False)
+    n0 = CF.max_threads()
+    try:
+        outs = []
+        for t in (1, 3, 8):
+            CF.set_threads(t)
+            outs.append(CF.srk2_steps(W, cm, 1e-3, 2, **kw))
+    finally:
+        CF.set_threads(n0)
+    assert _bits_equal(outs[0], outs[1]) and _bits_equal(outs[0], outs[2])
+
+
+def test_refuses_what_it_does_not_restate(CF):
+    from oracle import fvm as F
+    mesh = mesh_of("diamond", 0.2)
+    W, inlet = perturbed_state(mesh, base_cfg())
+    cm = CF.CMesh(F.omesh(mesh, np.float32))
+    with pytest.raises(ValueError):
+        CF.residual(W, cm, flux="ROE", value=inlet)
+    with pytest.raises(ValueError):
+        CF.residual(W, cm, flux="HLLC", entropy=True, value=inlet)
+    with pytest.raises(TypeError):
+        CF.CMesh(F.omesh(mesh, np.float64))
