@@ -20,21 +20,22 @@ Set-up outside the timed region (times reported): the reference fields, per
 geometry 55 solver-generated LR (h = 0.1) / HR training pairs (none of them
 a test case) and the training of one warm-start network per geometry.
 
-A "step" of this benchmark is one warm-start campaign over the 256 cases:
+A "step" of this benchmark is one warm-start campaign over the 1,024 cases:
 LR fixed-time solves of all cases in one solver, kNN/IDW features and MLP
 inference on the GPU, HR states with the freestream fallback, then the HR
-solves with continuous batching -- 128 cases in flight, each slot refilled
-with the next waiting case of its kind as soon as its case is done
-(esr_solver_replace_case). ``value`` = converged cases per hour over the K
-timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
+solves with continuous batching -- about 64 cases in flight in launch
+sub-groups of 6 cases of one kind, each slot refilled with the next waiting
+case of its kind as soon as its case is done (esr_solver_replace_case), idle
+sub-groups handed to the kinds with waiting cases (pipeline.campaign).
+``value`` = converged cases per hour over the K timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
 the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
 the warm start against those plain solves, the same campaign from cold
-starts, the stage kernels' HBM rates at the campaign's 128-case launch, and the
+starts, the stage kernels' HBM rates at the campaign's 64-case launch, and the
 reference's single-case configuration (6 cases: diamond + NACA0012, M in
 {0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
 for the slowest stage kernel of the autotuned launch of the campaign's
-128-case batch, timed with HIP events on the solver's stream over
-back-to-back launches; its
+64-case batch, timed with HIP events on the solver's stream between the
+kernels of real stage sequences; its
 algorithmic bytes per launch come from the launch's work-items
 (``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
 
@@ -104,14 +105,13 @@ def traffic_file(tag):
     return os.path.join(ROOT, "profiles", f"traffic_bench_{tag}.txt")
 
 
-def measured_traffic(kernel, tuning, tag, n_classes=1):
-    """HBM bytes per launch of `kernel` over the whole launch set (one
-    dispatch per kernel class, n_classes of them) from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (per-dispatch
-    means, calibrated against a 1 GiB copy), when that summary was taken with
-    the same launch configuration; else None. Kernels templated on the flux
-    (k_face, k_cell) have one instance per class; k_recon and k_update<2>
-    are one instance dispatched once per class."""
+def measured_traffic(kernel, tuning, tag):
+    """HBM bytes per launch set of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (per-dispatch means,
+    calibrated against a 1 GiB copy), when that summary was taken with the
+    same launch configuration; else None. Kernels templated on the flux
+    (k_face, k_cell) are dispatched once per flux class (their instances are
+    summed); k_recon and k_update are one dispatch over every group."""
     path = traffic_file(tag)
     if not os.path.exists(path):
         return None
@@ -124,7 +124,7 @@ def measured_traffic(kernel, tuning, tag, n_classes=1):
             if (name.startswith(pre) if pre.endswith("<") else name == pre)]
     if not hits:
         return None
-    return sum(hits) if pre.endswith("<") else hits[0] * n_classes
+    return sum(hits) if pre.endswith("<") else hits[0]
 
 
 def kernel_table(sizes, ms):
@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--cases", type=int, default=1024, help="cases of the ragged warm-start batch")
-    ap.add_argument("--slots", type=int, default=128, help="cases in flight (continuous batching)")
+    ap.add_argument("--slots", type=int, default=64, help="cases in flight (continuous batching)")
     ap.add_argument("--single-sample", type=int, default=16,
                     help="cases of the batch run alone by the plain single-case solver")
     ap.add_argument("--h", type=float, default=0.025)
@@ -470,8 +470,7 @@ def main():
                    "cases": args.cases, "h": args.h, "h_lr": args.h_lr, "cells_total": int(sum(cells))},
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["gbs"] / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, tuning, f"{args.slots}slot_h{args.h}",
-                                                 len({c.hr_spec.kw["flux"] for c in cases})),
+                     "traffic": measured_traffic(dom, tuning, f"{args.slots}slot_h{args.h}"),
                      "kernel": dom, "stage_path": tuning["path"], "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch"]},
         "speedup_vs_plain_single_case": value / single_rate if single_rate > 0 else None,
@@ -503,7 +502,7 @@ def main():
         "single_case_config": small,
     }
     if not args.no_cpu_baseline:
-        log("cpu baseline (numpy oracle)")
+        log("cpu baseline (C restatement of the oracle)")
         conv_steps = [p["stopping_step"] for p in (single or []) if p["converged"]]
         mean_steps = float(np.mean(conv_steps)) if conv_steps else 15000.0
         dia = next(c for c in cases if c.geometry == "diamond")

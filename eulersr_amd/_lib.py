@@ -45,7 +45,7 @@ class Case(C.Structure):
         ("q_ref", C.c_double), ("track_engineering", C.c_int32),
         ("target_cd", C.c_double), ("target_cl", C.c_double), ("cd_tol_abs", C.c_double),
         ("cl_tol_abs", C.c_double), ("engineering_patience", C.c_int32), ("threshold", C.c_double),
-        ("max_steps", C.c_int64),
+        ("max_steps", C.c_int64), ("group", C.c_int32),
     ]
 
 

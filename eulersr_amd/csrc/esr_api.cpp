@@ -87,6 +87,7 @@ std::vector<Stage> stages_of(int scheme) {
 struct CaseSpec {
   const esr_mesh* mesh;
   esr_opts opts;
+  int32_t tag = 0;  // esr_case.group: different tags never share a group
 };
 
 }  // namespace
@@ -124,7 +125,26 @@ struct Engine {
     DevBuf blk, fblk;
     std::vector<int32_t> gids;  // groups of the class
   };
-  std::vector<std::unique_ptr<Cls>> classes;
+  std::vector<std::unique_ptr<Cls>> classes;  // per (flux, recon, entropy): the flux kernels
+  Cls rec_all;   // every MUSCL group: the reconstruction (flux-independent), one launch
+  Cls cell_all;  // every group: the update and the commit (flux-independent), one launch
+  DevBuf red_tbl, frc_tbl;        // reduction block tables over the launched groups
+  int32_t n_red = 0, n_frc = 0;
+
+  // norms / forces block tables of the launched groups
+  void reduction_tables() {
+    std::vector<RedRef> red;
+    std::vector<BlockRef> frc;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+      if (!on(gi)) continue;
+      red_table(groups[gi], (int32_t)gi, red);
+      for (int32_t c = 0; c < groups[gi].C; ++c) frc.push_back({(int32_t)gi, c});
+    }
+    upload(red_tbl, red);
+    upload(frc_tbl, frc);
+    n_red = (int32_t)red.size();
+    n_frc = (int32_t)frc.size();
+  }
   bool any_muscl = false, any_lf = false;
   int64_t elems = 0, rec_elems = 0, fx_elems = 0;
   DevBuf W0, W1, W2, acc, Wref, rec, fx;
@@ -141,28 +161,24 @@ struct Engine {
   // cell / face block tables of a class for `block` threads per block
   // (host group copies updated; the caller uploads groups_d)
   void block_tables(Cls& c, int32_t block, int32_t xcd = 1) {
-    std::vector<int32_t> blk, fblk;
+    std::vector<BlockRef> blk, fblk;
     for (int32_t gi : c.gids) {
       if (!group_on.empty() && !group_on[gi]) continue;  // every case of the group is done
-      GroupDev& g = groups[gi];
-      const int64_t items = (int64_t)g.m.n * g.C;
-      g.blk0 = (int32_t)blk.size();
-      g.nblk = (int32_t)((items + block - 1) / block);
-      for (int32_t b = 0; b < g.nblk; ++b) blk.push_back(gi);
-      const int64_t fitems = (int64_t)g.m.nf * g.C;
-      g.fblk0 = (int32_t)fblk.size();
-      g.fnblk = (int32_t)((fitems + block - 1) / block);
-      for (int32_t b = 0; b < g.fnblk; ++b) fblk.push_back(gi);
+      const GroupDev& g = groups[gi];
+      const int64_t nblk = ((int64_t)g.m.n * g.C + block - 1) / block;
+      for (int32_t b = 0; b < (int32_t)nblk; ++b) blk.push_back({gi, b});
+      const int64_t fnblk = ((int64_t)g.m.nf * g.C + block - 1) / block;
+      for (int32_t b = 0; b < (int32_t)fnblk; ++b) fblk.push_back({gi, b});
     }
-    // launches cover a multiple of 8 blocks (one run per XCD); padding -1
-    while (blk.size() % 8) blk.push_back(-1);
-    while (fblk.size() % 8) fblk.push_back(-1);
+    // launches cover a multiple of 8 blocks (one run per XCD); padding group -1
+    while (blk.size() % 8) blk.push_back({-1, 0});
+    while (fblk.size() % 8) fblk.push_back({-1, 0});
     const int32_t nb = (int32_t)blk.size(), nfb = (int32_t)fblk.size();
     upload(c.blk, blk);
     upload(c.fblk, fblk);
-    c.ls.blk_group = c.blk.as<int32_t>();
+    c.ls.blk_group = c.blk.as<BlockRef>();
     c.ls.nblocks = nb;
-    c.ls.fblk_group = c.fblk.as<int32_t>();
+    c.ls.fblk_group = c.fblk.as<BlockRef>();
     c.ls.fnblocks = nfb;
     c.ls.block = block;
     c.ls.xcd = xcd ? 1 : 0;
@@ -172,8 +188,9 @@ struct Engine {
   void configure(int32_t path, int32_t block, int32_t xcd, hipStream_t st) {
     hip_check(hipStreamSynchronize(st), "configure sync");
     for (auto& c : classes) block_tables(*c, block, xcd);
-    hip_check(hipMemcpy(groups_d.p, groups.data(), groups.size() * sizeof(GroupDev), hipMemcpyHostToDevice),
-              "groups H2D");
+    block_tables(rec_all, block, xcd);
+    block_tables(cell_all, block, xcd);
+    reduction_tables();
     stage_path = path;
     cur_block = block;
     cur_xcd = xcd;
@@ -202,7 +219,7 @@ struct Engine {
     (void)stages_of(scheme);
     n_cases = (int32_t)specs.size();
     if (n_cases <= 0) throw Error(ESR_ERR_VALUE, "no cases");
-    using Key = std::tuple<const esr_mesh*, int, int, int, double, double, float>;
+    using Key = std::tuple<const esr_mesh*, int, int, int, double, double, float, int32_t>;
     std::map<Key, std::vector<int32_t>> gm;
     std::vector<Key> order;
     for (int32_t i = 0; i < n_cases; ++i) {
@@ -210,7 +227,7 @@ struct Engine {
       if (!s.mesh) throw Error(ESR_ERR_VALUE, "case without a mesh");
       validate_opts(&s.opts);
       Key k{s.mesh, s.opts.flux, s.opts.recon, s.opts.entropy ? 1 : 0, s.opts.gamma, s.opts.M,
-            s.opts.sponge_width > 0 ? s.opts.sponge_width : -1.0f};
+            s.opts.sponge_width > 0 ? s.opts.sponge_width : -1.0f, s.tag};
       if (!gm.count(k)) order.push_back(k);
       gm[k].push_back(i);
     }
@@ -251,6 +268,7 @@ struct Engine {
         g.sponge_g = m->dev.sponge_g;
       }
       st += (int64_t)m->n * g.C;
+      g.flux = std::get<1>(k);
       g.fx_off = fo;
       fo += (int64_t)m->dev.nf * g.C;
       if (std::get<2>(k) == ESR_RECON_MUSCL) {
@@ -273,11 +291,14 @@ struct Engine {
     elems = st;
     rec_elems = rc;
     fx_elems = fo;
-    // launch classes
+    // launch classes of the flux kernels: HLLC and Roe groups share one
+    // (the flux is picked per group inside the kernel)
     std::map<std::tuple<int, int, int>, std::vector<int32_t>> cls;
     for (size_t gi = 0; gi < order.size(); ++gi) {
       const Key& k = order[gi];
-      cls[{std::get<1>(k), std::get<2>(k), std::get<3>(k)}].push_back((int32_t)gi);
+      const int fl = std::get<1>(k);
+      const int lc = (fl == ESR_FLUX_HLLC || fl == ESR_FLUX_ROE) ? FLUX_HLLC_ROE : fl;
+      cls[{lc, std::get<2>(k), std::get<3>(k)}].push_back((int32_t)gi);
     }
     for (auto& kv : cls) {
       auto c = std::make_unique<Cls>();
@@ -288,8 +309,18 @@ struct Engine {
       block_tables(*c, BLOCK);
       classes.push_back(std::move(c));
     }
+    for (size_t gi = 0; gi < order.size(); ++gi) {
+      if (std::get<2>(order[gi]) == ESR_RECON_MUSCL) rec_all.gids.push_back((int32_t)gi);
+      cell_all.gids.push_back((int32_t)gi);
+    }
+    rec_all.ls.recon = ESR_RECON_MUSCL;
+    block_tables(rec_all, BLOCK);
+    block_tables(cell_all, BLOCK);
     upload(groups_d, groups);
     for (auto& c : classes) c->ls.groups = groups_d.as<GroupDev>();
+    rec_all.ls.groups = groups_d.as<GroupDev>();
+    cell_all.ls.groups = groups_d.as<GroupDev>();
+    reduction_tables();
 
     // case params (user order -> global order)
     cases_h.assign(n_cases, CaseDev{});
@@ -341,8 +372,7 @@ struct Engine {
 
   // one stage: reconstruction, face fluxes, gather + update into `out`
   void stage(int mode, const float4* Ws, float4* out, hipStream_t st) {
-    for (auto& c : classes)
-      if (c->ls.recon == ESR_RECON_MUSCL) launch_recon(c->ls, cases(), Ws, rec.as<float4>(), st);
+    launch_recon(rec_all.ls, cases(), Ws, rec.as<float4>(), st);
     for (auto& c : classes) {
       if (c->ls.flux == ESR_FLUX_LF) {
         launch_zero_lf(n_cases, cases(), st);
@@ -350,19 +380,18 @@ struct Engine {
       }
       if (stage_path == ESR_STAGE_FACE) launch_face(c->ls, cases(), Ws, rec.as<float4>(), fx.as<float4>(), st);
     }
-    for (auto& c : classes) {
-      if (stage_path == ESR_STAGE_FACE)
-        launch_update(c->ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
-      else
+    if (stage_path == ESR_STAGE_FACE)
+      launch_update(cell_all.ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
+    else
+      for (auto& c : classes)
         launch_cell(c->ls, mode, cases(), Ws, W0.as<float4>(), rec.as<float4>(), out, acc.as<float4>(), st);
-    }
   }
 
   // one Runge-Kutta step of every active case
   void step(hipStream_t st) {
     for (const Stage& sg : stages_of(scheme)) stage(sg.mode, buf(sg.ws), buf(sg.out), st);
     if (scheme == ESR_SCHEME_EE)
-      for (auto& c : classes) launch_commit(c->ls, cases(), W1.as<float4>(), W0.as<float4>(), st);
+      launch_commit(cell_all.ls, cases(), W1.as<float4>(), W0.as<float4>(), st);
   }
 
   // residual of W0 into W1 (one case engines)
@@ -419,18 +448,13 @@ struct Engine {
 
   // norms of (Wn - Wref) and Wref for cases flagged 'check'
   // fused: the check arguments, run by the last block of the last launch
+  // one launch over every launched group
   void norms(const float4* Wn, float4* Wref_, int copy_ref, hipStream_t st, const CheckArgs* fused = nullptr) {
-    size_t last = groups.size();
-    for (size_t gi = 0; gi < groups.size(); ++gi)
-      if (on(gi)) last = gi;
-    for (size_t gi = 0; gi < groups.size(); ++gi)
-      if (on(gi))
-        launch_norms(groups[gi], gdev((int32_t)gi), cases(), Wn, Wref_, part_d.as<double>(), part_r.as<double>(),
-                     rstride, copy_ref, st, gi == last ? fused : nullptr);
+    launch_norms(groups_d.as<GroupDev>(), red_tbl.as<RedRef>(), n_red, cases(), Wn, Wref_, part_d.as<double>(),
+                 part_r.as<double>(), rstride, copy_ref, st, fused);
   }
   void forces(const float4* W, hipStream_t st) {
-    for (size_t gi = 0; gi < groups.size(); ++gi)
-      if (on(gi)) launch_forces(groups[gi], gdev((int32_t)gi), cases(), W, forces_d.as<double>(), st);
+    launch_forces(groups_d.as<GroupDev>(), frc_tbl.as<BlockRef>(), n_frc, cases(), W, forces_d.as<double>(), st);
   }
 };
 
@@ -759,7 +783,7 @@ esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_s
     s->so = *so;
     s->cases.assign(cases, cases + n_cases);
     std::vector<CaseSpec> specs;
-    for (int32_t i = 0; i < n_cases; ++i) specs.push_back(CaseSpec{cases[i].mesh, cases[i].opts});
+    for (int32_t i = 0; i < n_cases; ++i) specs.push_back(CaseSpec{cases[i].mesh, cases[i].opts, cases[i].group});
     s->eng.build(specs, so->scheme);
     s->dt.assign(n_cases, 0.f);
     s->N.assign(n_cases, 0);
@@ -1120,15 +1144,18 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
     hipStream_t st = S(stream);
     Engine& e = s->eng;
     if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
-    // launch shape of the solver: per class its kernel configuration and sizes
+    // launch shape of the solver: per class its kernel configuration and the
+    // sizes of its launched groups
     std::vector<int64_t> key{(int64_t)e.scheme};
     for (auto& c : e.classes) {
-      int64_t items = 0, fitems = 0;
+      int64_t items = 0, fitems = 0, ng = 0;
       for (int32_t gi : c->gids) {
+        if (!e.on(gi)) continue;
         items += (int64_t)e.groups[gi].m.n * e.groups[gi].C;
         fitems += (int64_t)e.groups[gi].m.nf * e.groups[gi].C;
+        ++ng;
       }
-      key.insert(key.end(), {c->ls.flux, c->ls.recon, c->ls.entropy, (int64_t)c->gids.size(), items, fitems});
+      key.insert(key.end(), {c->ls.flux, c->ls.recon, c->ls.entropy, ng, items, fitems});
     }
     TuneChoice best{};
     bool cached = false;
@@ -1144,11 +1171,15 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
       launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
       hipEvent_t ev[2];
       for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
-      best.ms = INFINITY;
-      for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE}) {
-        for (int32_t b : {64, 128, 256}) {
-         for (int32_t x : {1, 0}) {
-          e.configure(p, b, x, st);
+      // every configuration timed in two sweeps (the best of the two counts),
+      // so that clock ramps and neighbours' noise do not pick a slower one
+      std::vector<TuneChoice> cand;
+      for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE})
+        for (int32_t b : {64, 128, 256})
+          for (int32_t x : {1, 0}) cand.push_back(TuneChoice{p, b, x, INFINITY});
+      for (int sweep = 0; sweep < 2; ++sweep) {
+        for (TuneChoice& c : cand) {
+          e.configure(c.path, c.block, c.xcd, st);
           e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);  // code objects loaded
           hip_check(hipEventRecord(ev[0], st), "record");
           for (int32_t r = 0; r < reps; ++r) e.stage(MODE_S1, e.W0.as<float4>(), e.W1.as<float4>(), st);
@@ -1156,11 +1187,12 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
           hip_check(hipEventSynchronize(ev[1]), "event sync");
           float t = 0.f;
           hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "elapsed");
-          const double ms = (double)t / reps;
-          if (ms < best.ms) best = TuneChoice{p, b, x, ms};
-         }
+          c.ms = std::min(c.ms, (double)t / reps);
         }
       }
+      best.ms = INFINITY;
+      for (const TuneChoice& c : cand)
+        if (c.ms < best.ms) best = c;
       for (auto& x : ev) (void)hipEventDestroy(x);
       launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
       std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -1210,21 +1242,14 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
     hipStream_t st = S(stream);
     Engine& e = s->eng;
     launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
-    hipEvent_t ev[5];
-    for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
-    auto recon = [&] {
-      for (auto& cl : e.classes)
-        if (cl->ls.recon == ESR_RECON_MUSCL)
-          launch_recon(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st);
-    };
+    auto recon = [&] { launch_recon(e.rec_all.ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st); };
     auto face = [&] {
       for (auto& cl : e.classes) launch_face(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(),
                                              e.fx.as<float4>(), st);
     };
     auto update = [&] {
-      for (auto& cl : e.classes)
-        launch_update(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.fx.as<float4>(),
-                      e.W1.as<float4>(), e.acc.as<float4>(), st);
+      launch_update(e.cell_all.ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.fx.as<float4>(),
+                    e.W1.as<float4>(), e.acc.as<float4>(), st);
     };
     auto cell = [&] {
       for (auto& cl : e.classes)
@@ -1235,21 +1260,37 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
     face();
     update();
     cell();  // code objects loaded, buffers valid
-    hip_check(hipEventRecord(ev[0], st), "record");
-    for (int32_t r = 0; r < reps; ++r) recon();
-    hip_check(hipEventRecord(ev[1], st), "record");
-    for (int32_t r = 0; r < reps; ++r) face();
-    hip_check(hipEventRecord(ev[2], st), "record");
-    for (int32_t r = 0; r < reps; ++r) update();
-    hip_check(hipEventRecord(ev[3], st), "record");
-    for (int32_t r = 0; r < reps; ++r) cell();
-    hip_check(hipEventRecord(ev[4], st), "record");
-    hip_check(hipEventSynchronize(ev[4]), "event sync");
-    for (int q = 0; q < 4; ++q) {
-      float t = 0.f;
-      hip_check(hipEventElapsedTime(&t, ev[q], ev[q + 1]), "elapsed");
-      ms[q] = (double)t / reps;
+    // Each kernel timed inside the stage sequence it belongs to (face path:
+    // recon, face, update; cell path: recon, cell), with an event between
+    // kernels, so every launch meets the cache state of a real stage.
+    std::vector<hipEvent_t> ev(6 * (size_t)reps);
+    for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
+    for (int32_t r = 0; r < reps; ++r) {
+      hipEvent_t* q = ev.data() + 6 * (size_t)r;
+      hip_check(hipEventRecord(q[0], st), "record");
+      recon();
+      hip_check(hipEventRecord(q[1], st), "record");
+      face();
+      hip_check(hipEventRecord(q[2], st), "record");
+      update();
+      hip_check(hipEventRecord(q[3], st), "record");
+      recon();
+      hip_check(hipEventRecord(q[4], st), "record");
+      cell();
+      hip_check(hipEventRecord(q[5], st), "record");
     }
+    hip_check(hipEventSynchronize(ev.back()), "event sync");
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int32_t r = 0; r < reps; ++r) {
+      hipEvent_t* q = ev.data() + 6 * (size_t)r;
+      float t = 0.f;
+      const int pairs[4][2] = {{0, 1}, {1, 2}, {2, 3}, {4, 5}};
+      for (int k = 0; k < 4; ++k) {
+        hip_check(hipEventElapsedTime(&t, q[pairs[k][0]], q[pairs[k][1]]), "elapsed");
+        acc[k] += (double)t;
+      }
+    }
+    for (int k = 0; k < 4; ++k) ms[k] = acc[k] / reps;
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (s->prepared) launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
   });

@@ -100,13 +100,28 @@ struct GroupDev {
   int64_t st_off;        // float4 offset of the group's state slab
   int64_t rec_off;       // float4 offset of the group's reconstruction slab ([3][n][C])
   int64_t fx_off;        // float4 offset of the group's face-flux slab ([nf][C])
-  int32_t blk0;          // first cell block of the group inside its launch
-  int32_t nblk;          // cell blocks of the group
-  int32_t fblk0;         // first face block of the group inside its launch
-  int32_t fnblk;         // face blocks of the group
+  int32_t flux;          // esr_flux of the group
   Phys ph;
   int32_t sponge;
   const float* sponge_g; // mesh default or a custom width
+};
+
+// One entry of a launch's block table: the group a block works on and the
+// block's index inside that group (group -1: padding). Any subset of groups
+// can form a launch: the flux kernels launch per flux class, the
+// reconstruction and the update once over all groups.
+struct BlockRef {
+  int32_t group;
+  int32_t block;
+};
+
+// One block of a reduction launch over several groups: the group, its cell
+// range (blockIdx.x of a one-group launch) and case chunk (blockIdx.y).
+struct RedRef {
+  int32_t group;
+  int32_t range;
+  int32_t chunk;
+  int32_t pad;
 };
 
 // Per-case parameters read by the stage kernels.

@@ -113,15 +113,14 @@ __device__ __forceinline__ int32_t logical_block(int xcd) {
 }
 
 template <bool FACES>
-__device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const int32_t* __restrict__ blk_group,
+__device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const BlockRef* __restrict__ blk_group,
                                          const CaseDev* __restrict__ cases, int xcd, Item& it) {
-  const int32_t lb = logical_block(xcd);
-  const int32_t gi = blk_group[lb];
-  if (gi < 0) return false;
-  const GroupDev* G = groups + gi;
+  const BlockRef e = blk_group[logical_block(xcd)];
+  if (e.group < 0) return false;
+  const GroupDev* G = groups + e.group;
   const int32_t C = G->C;
   // t / C by the group's magic number (in-group indices are 32-bit)
-  const uint32_t t = (uint32_t)(lb - (FACES ? G->fblk0 : G->blk0)) * blockDim.x + threadIdx.x;
+  const uint32_t t = (uint32_t)e.block * blockDim.x + threadIdx.x;
   uint32_t q = t;
   if (C > 1) {
     const uint32_t h = __umulhi(G->cmagic, t);
@@ -159,7 +158,7 @@ __device__ __forceinline__ float venkat(float a, float b) {
 }
 
 __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ groups,
-                                                 const int32_t* __restrict__ blk_group,
+                                                 const BlockRef* __restrict__ blk_group,
                                                  const CaseDev* __restrict__ cases,
                                                  const float4* __restrict__ Ws, float4* __restrict__ rec, int xcd) {
   Item it;
@@ -618,9 +617,24 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
 // per-face flux: F * len of each logical face, owner's orientation
 // ---------------------------------------------------------------------------
 
+// face_flux of a launch class: FLUX_HLLC_ROE picks HLLC or Roe by the
+// group's flux (uniform over a block)
+template <int FLUX, int ENT>
+__device__ __forceinline__ void class_flux(int32_t gflux, const float4 WL, const float4 WR, float nx, float ny,
+                                           const Phys& ph, float lf_c, float F[4]) {
+  if (FLUX == FLUX_HLLC_ROE) {
+    if (gflux == ESR_FLUX_ROE)
+      face_flux<ESR_FLUX_ROE, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+    else
+      face_flux<ESR_FLUX_HLLC, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+  } else {
+    face_flux<FLUX, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+  }
+}
+
 template <int RECON, int FLUX, int ENT>
 __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ groups,
-                                                const int32_t* __restrict__ blk_group,
+                                                const BlockRef* __restrict__ blk_group,
                                                 const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
                                                 const float4* __restrict__ rec, float4* __restrict__ fx, int xcd) {
   Item it;
@@ -634,7 +648,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
   const float sk = m.f_len[it.idx];
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   float F[4];
-  face_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, G->ph, lf_c, F);
+  class_flux<FLUX, ENT>(G->flux, WL, WR, nk.x, nk.y, G->ph, lf_c, F);
   fx[fidx_of(G, it.idx, it.c)] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
 }
 
@@ -642,7 +656,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
 // (euler_solver.py:267-269), integer max on the float bits (values >= 0).
 template <int RECON>
 __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ groups,
-                                                   const int32_t* __restrict__ blk_group, CaseDev* cases,
+                                                   const BlockRef* __restrict__ blk_group, CaseDev* cases,
                                                    const float4* __restrict__ Ws, const float4* __restrict__ rec, int xcd) {
   Item it;
   if (!map_item<true>(groups, blk_group, cases, xcd, it)) return;
@@ -718,7 +732,7 @@ __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, cons
 
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ groups,
-                                                  const int32_t* __restrict__ blk_group,
+                                                  const BlockRef* __restrict__ blk_group,
                                                   const CaseDev* __restrict__ cases,
                                                   const float4* __restrict__ Ws, const float4* W0,
                                                   const float4* __restrict__ fx, float4* out, float4* acc, int xcd) {
@@ -747,7 +761,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
 
 template <int RECON, int FLUX, int ENT>
 __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ groups,
-                                                const int32_t* __restrict__ blk_group,
+                                                const BlockRef* __restrict__ blk_group,
                                                 const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
                                                 const float4* W0, const float4* __restrict__ rec, float4* out,
                                                 float4* acc, int mode, int xcd) {
@@ -787,7 +801,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
       WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
     }
     float F[4];
-    face_flux<FLUX, ENT>(WL, WR, nk.x, nk.y, ph, lf_c, F);
+    class_flux<FLUX, ENT>(G->flux, WL, WR, nk.x, nk.y, ph, lf_c, F);
     Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
   }
   stage_epilogue(mode, G, cs, cell, it.sidx, Wi, Fk, W0, out, acc);
@@ -816,15 +830,21 @@ int32_t reduce_ranges(const GroupDev& g) {
   return (g.m.n + cpr - 1) / cpr;
 }
 
-__device__ __forceinline__ RedMap red_map(const GroupDev* G) {
+__device__ __forceinline__ RedMap red_map(const GroupDev* G, int32_t range, int32_t chunk) {
   RedMap r;
-  r.c0 = blockIdx.y * BLOCK;
+  r.c0 = chunk * BLOCK;
   r.Ck = min(G->C - r.c0, BLOCK);
   r.P = BLOCK / r.Ck;
   const int32_t cpr = red_cells_per_range(G->C);
-  r.cell_begin = blockIdx.x * cpr;
+  r.cell_begin = range * cpr;
   r.cell_end = min(r.cell_begin + cpr, G->m.n);
   return r;
+}
+
+void red_table(const GroupDev& g, int32_t gi, std::vector<RedRef>& out) {
+  const int32_t nr = reduce_ranges(g), nc = (g.C + BLOCK - 1) / BLOCK;
+  for (int32_t y = 0; y < nc; ++y)
+    for (int32_t x = 0; x < nr; ++x) out.push_back({gi, x, y, 0});
 }
 
 __device__ void check_cases(int32_t c0, int32_t c1, CaseDev* cases, CaseLoop* loops, const double* part_d,
@@ -834,12 +854,14 @@ __device__ void check_cases(int32_t c0, int32_t c1, CaseDev* cases, CaseLoop* lo
 // ca.counter != null: the last block of this launch to finish (every earlier
 // launch of the stream has completed) runs the stationarity check of every
 // case, then resets the counter -- one launch less per check.
-__global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ gdev, const CaseDev* __restrict__ cases,
+__global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ groups, const RedRef* __restrict__ tbl,
+                                                 const CaseDev* __restrict__ cases,
                                                  const float4* __restrict__ Wn, float4* __restrict__ Wref,
                                                  double* __restrict__ part_d, double* __restrict__ part_r,
                                                  int32_t rstride, int copy_ref, CheckArgs ca) {
-  const GroupDev* G = gdev;
-  const RedMap r = red_map(G);
+  const RedRef e = tbl[blockIdx.x];
+  const GroupDev* G = groups + e.group;
+  const RedMap r = red_map(G, e.range, e.chunk);
   __shared__ double sd[BLOCK], sr[BLOCK];
   const int t = threadIdx.x;
   double accd = 0.0, accr = 0.0;
@@ -870,14 +892,14 @@ __global__ __launch_bounds__(BLOCK) void k_norms(const GroupDev* __restrict__ gd
       a += sd[q * r.Ck + t];
       b += sr[q * r.Ck + t];
     }
-    part_d[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = a;
-    part_r[(int64_t)(G->case0 + cc) * rstride + blockIdx.x] = b;
+    part_d[(int64_t)(G->case0 + cc) * rstride + e.range] = a;
+    part_r[(int64_t)(G->case0 + cc) * rstride + e.range] = b;
   }
   if (!ca.counter) return;
   __shared__ int last;
   __threadfence();
   __syncthreads();
-  if (t == 0) last = (atomicAdd(ca.counter, 1u) == gridDim.x * gridDim.y - 1) ? 1 : 0;
+  if (t == 0) last = (atomicAdd(ca.counter, 1u) == gridDim.x - 1) ? 1 : 0;
   __syncthreads();
   if (!last) return;
   __threadfence();  // agent-scope acquire: the other blocks' partials are visible
@@ -897,7 +919,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt(const GroupDev* __restrict__ gdev,
                                               const float4* __restrict__ W, float* __restrict__ part,
                                               int32_t rstride) {
   const GroupDev* G = gdev;
-  const RedMap r = red_map(G);
+  const RedMap r = red_map(G, blockIdx.x, blockIdx.y);
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
   __shared__ float sm[BLOCK];
@@ -934,12 +956,14 @@ __global__ __launch_bounds__(BLOCK) void k_dt(const GroupDev* __restrict__ gdev,
 
 // CD/CL wall sums in FP64 (helper.py:366-415). Pressure with the default
 // gamma = 1.4 and M = 1. One block per case of the group.
-__global__ __launch_bounds__(BLOCK) void k_forces(const GroupDev* __restrict__ gdev,
+__global__ __launch_bounds__(BLOCK) void k_forces(const GroupDev* __restrict__ groups,
+                                                  const BlockRef* __restrict__ tbl,
                                                   const CaseDev* __restrict__ cases, const float4* __restrict__ W,
                                                   double* __restrict__ cd_cl) {
-  const GroupDev* G = gdev;
+  const BlockRef e = tbl[blockIdx.x];
+  const GroupDev* G = groups + e.group;
   const MeshDev& m = G->m;
-  const int32_t c = blockIdx.x;
+  const int32_t c = e.block;
   __shared__ double sx[BLOCK], sy[BLOCK];
   double ax = 0.0, ay = 0.0;
   if (cases[G->case0 + c].check) {
@@ -1161,7 +1185,7 @@ __global__ void k_probe_final(int32_t n_cases, CaseDev* cases, const double* __r
 }
 
 __global__ __launch_bounds__(BLOCK) void k_commit(const GroupDev* __restrict__ groups,
-                                                  const int32_t* __restrict__ blk_group,
+                                                  const BlockRef* __restrict__ blk_group,
                                                   const CaseDev* __restrict__ cases, const float4* __restrict__ src,
                                                   float4* __restrict__ dst, int xcd) {
   Item it;
@@ -1222,6 +1246,10 @@ static void launch_face_f(const LaunchSet& ls, const CaseDev* cases, const float
       hipLaunchKernelGGL((k_face<RECON, ESR_FLUX_ROE, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws, rec,
                          fx, ls.xcd);
       break;
+    case FLUX_HLLC_ROE:
+      hipLaunchKernelGGL((k_face<RECON, FLUX_HLLC_ROE, ENT>), g, b, 0, st, ls.groups, ls.fblk_group, cases, Ws,
+                         rec, fx, ls.xcd);
+      break;
     default: throw Error(ESR_ERR_VALUE, "unknown flux");
   }
 }
@@ -1259,6 +1287,7 @@ static void launch_cell_f(const LaunchSet& ls, int mode, const CaseDev* cases, c
     ESR_CELL_CASE(ESR_FLUX_AUSM)
     ESR_CELL_CASE(ESR_FLUX_HLLC)
     ESR_CELL_CASE(ESR_FLUX_ROE)
+    ESR_CELL_CASE(FLUX_HLLC_ROE)
     default: throw Error(ESR_ERR_VALUE, "unknown flux");
   }
 #undef ESR_CELL_CASE
@@ -1311,13 +1340,14 @@ static dim3 red_grid(const GroupDev& g) {
   return dim3((unsigned)reduce_ranges(g), (unsigned)((g.C + BLOCK - 1) / BLOCK));
 }
 
-void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn, float4* Wref,
-                  double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st,
+void launch_norms(const GroupDev* groups, const RedRef* tbl, int32_t n, const CaseDev* cases, const float4* Wn,
+                  float4* Wref, double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st,
                   const CheckArgs* fused) {
+  if (n <= 0) return;
   CheckArgs ca{};
   if (fused) ca = *fused;
-  hipLaunchKernelGGL(k_norms, red_grid(g), dim3(BLOCK), 0, st, gdev, cases, Wn, Wref, part_d, part_r, rstride,
-                     copy_ref, ca);
+  hipLaunchKernelGGL(k_norms, dim3((unsigned)n), dim3(BLOCK), 0, st, groups, tbl, cases, Wn, Wref, part_d, part_r,
+                     rstride, copy_ref, ca);
   check_launch("k_norms");
 }
 
@@ -1327,9 +1357,10 @@ void launch_dt(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, co
   check_launch("k_dt");
 }
 
-void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, double* cd_cl,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(k_forces, dim3((unsigned)g.C), dim3(BLOCK), 0, st, gdev, cases, W, cd_cl);
+void launch_forces(const GroupDev* groups, const BlockRef* tbl, int32_t n, const CaseDev* cases, const float4* W,
+                   double* cd_cl, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_forces, dim3((unsigned)n), dim3(BLOCK), 0, st, groups, tbl, cases, W, cd_cl);
   check_launch("k_forces");
 }
 

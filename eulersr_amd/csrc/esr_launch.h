@@ -2,16 +2,23 @@
 // host orchestration (esr_api.cpp). Internal.
 #pragma once
 
+#include <vector>
+
 #include "esr_internal.h"
 
 namespace esr {
 
+// Launch class of the flux kernels holding HLLC and Roe groups together: the
+// flux is chosen per group at run time (uniform over a block), so both
+// run in one launch.
+constexpr int32_t FLUX_HLLC_ROE = 100;
+
 // A launch covers a set of groups with the same kernel configuration.
 struct LaunchSet {
   const GroupDev* groups = nullptr;   // device array
-  const int32_t* blk_group = nullptr; // device array: cell block -> group index
+  const BlockRef* blk_group = nullptr;  // device array: cell block -> (group, block in group)
   int32_t nblocks = 0;
-  const int32_t* fblk_group = nullptr; // device array: face block -> group index
+  const BlockRef* fblk_group = nullptr; // device array: face block -> (group, block in group)
   int32_t fnblocks = 0;
   int32_t flux = ESR_FLUX_HLLC;
   int32_t recon = ESR_RECON_MUSCL;
@@ -48,12 +55,17 @@ struct CheckArgs {
   int64_t* dstep;
   uint32_t* counter;  // zero between launches; null: no fused check
 };
-void launch_norms(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* Wn,
+// Block tables of the reductions over a set of groups (host side).
+void red_table(const GroupDev& g, int32_t gi, std::vector<RedRef>& out);
+// One launch over every block of `tbl` (n entries): stationarity norms of
+// each case (and the fused check on the launch's last block), or the FP64
+// wall force sums (one block per case: BlockRef {group, case in group}).
+void launch_norms(const GroupDev* groups, const RedRef* tbl, int32_t n, const CaseDev* cases, const float4* Wn,
                   float4* Wref, double* part_d, double* part_r, int32_t rstride, int copy_ref, hipStream_t st,
                   const CheckArgs* fused = nullptr);
 void launch_dt(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W, float* part,
                int32_t rstride, int local, hipStream_t st);
-void launch_forces(const GroupDev& g, const GroupDev* gdev, const CaseDev* cases, const float4* W,
+void launch_forces(const GroupDev* groups, const BlockRef* tbl, int32_t n, const CaseDev* cases, const float4* W,
                    double* cd_cl, hipStream_t st);
 
 // Finalisation of a stationarity check for every case (main.py:149-177).

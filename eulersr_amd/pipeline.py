@@ -201,34 +201,60 @@ def stalled(hist, steps, after, window, factor):
     return bool(before) and bool(recent) and min(recent) > factor * min(before)
 
 
-def initial_slots(specs, slots):
-    """Per-kind queues of case ids and the cases that take the `slots` slots
-    first: slots split over the kinds in proportion to their case counts (at
-    least one each), cases in input order."""
+def _queues(specs):
     from collections import OrderedDict, deque
-    n = len(specs)
     queues = OrderedDict()
     for i, sp in enumerate(specs):
         queues.setdefault(_kind(sp), deque()).append(i)
-    total = min(int(slots), n)
-    share = {k: max(1, int(round(total * len(queues[k]) / n))) for k in queues}
-    while sum(share.values()) > total and any(v > 1 for v in share.values()):
+    return queues
+
+
+SUBGROUP_MAX = 16     # cases per launch sub-group of the campaign scheduler, at most
+LANE_FACTOR = 1.25    # launched slots (live sub-groups x their size) <= LANE_FACTOR x slots
+
+
+def subgroup_size(specs, slots):
+    """Cases per launch sub-group of the campaign scheduler: the open slots
+    split over the kinds (at least 1, at most SUBGROUP_MAX)."""
+    return max(1, min(SUBGROUP_MAX, int(slots) // max(1, len(_queues(specs)))))
+
+
+def initial_slots(specs, slots):
+    """Per-kind queues of case ids and the cases in flight at the start of a
+    campaign: `slots` // S sub-groups of S cases (subgroup_size) opened over
+    the kinds in proportion to their case counts (at least one each), cases
+    in input order."""
+    queues = _queues(specs)
+    n = len(specs)
+    S = subgroup_size(specs, slots)
+    n_open = max(1, int(slots) // S)
+    share = {k: max(1, int(round(n_open * len(queues[k]) / n))) for k in queues}
+    while sum(share.values()) > max(n_open, len(queues)) and any(v > 1 for v in share.values()):
         k = max(share, key=lambda q: share[q])
         share[k] -= 1
     first = []
     for k in queues:
-        for _ in range(min(share[k], len(queues[k]))):
+        for _ in range(min(share[k] * S, len(queues[k]))):
             first.append(queues[k].popleft())
     return queues, first
 
 
-def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL):
+def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL, trace=None):
     """Continuous batching: solves every case of `specs` (initial states
-    `inits`) to its own stationarity criterion with at most `slots` cases in
+    `inits`) to its own stationarity criterion with about `slots` cases in
     flight; whenever a case is done (converged, diverged or at its step cap
     `max_steps`), the next waiting case of the same kind (mesh and kernel
-    configuration) takes its slot (esr_solver_replace_case). Slots are split
-    over the kinds in proportion to their case counts.
+    configuration) takes its slot (esr_solver_replace_case).
+
+    Slots live in launch sub-groups of S cases of one kind (subgroup_size;
+    esr_case.group keeps them apart). Each kind holds enough sub-groups to
+    take every open slot, but only `slots` // S of them start open; a
+    sub-group whose cases are all done drops out of every launch
+    (esr_solver_compact), and while room is left (cases in flight and
+    launched slots, LANE_FACTOR) a whole idle sub-group of the kind with the
+    most waiting cases is opened. Kinds whose cases converge fast therefore
+    hand their share of the GPU to the slower kinds instead of leaving it
+    idle.
 
     fallback_inits (warm starts): a case whose solve diverges, or stalls (its
     best check residual no longer falls, `stalled`) above its threshold, is
@@ -236,30 +262,77 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     limiter a warm start can settle on a limit cycle whose residual stays
     above the threshold set by the cold trajectory.
 
+    trace: a list that receives (seconds, step, cases in flight, cases
+    waiting) after every chunk.
+
     Returns (reports in input order, seconds of solving, restarted case ids)."""
+    import math
     import torch
+    if chunk % CHECK_EVERY:
+        raise ValueError(f"chunk must be a multiple of the check period {CHECK_EVERY}")
     specs = [copy.copy(sp) for sp in specs]
     if max_steps:
         for sp in specs:
             sp.max_steps = int(max_steps)
     n = len(specs)
+    S = subgroup_size(specs, slots)
+    n_open = max(1, int(slots) // S)
     queues, first = initial_slots(specs, slots)
-    bs = BatchSolver([specs[i] for i in first], scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY,
-                     patience=PATIENCE, autotune=AUTOTUNE, launch=LAUNCH)
+    # sub-groups: (kind, slot ids); placeholders (one step, then done) fill
+    # the slots of sub-groups that start idle
+    start = {}
+    for ci in first:
+        start.setdefault(_kind(specs[ci]), []).append(ci)
+    sub_kind, sub_slots, slot_case, slot_spec, slot_init = [], [], [], [], []
+    tag = 0
+    for k in list(queues.keys()):
+        kind_cases = start.get(k, []) + list(queues[k])
+        n_sub = min(math.ceil(len(kind_cases) / S), n_open)
+        mine = start.get(k, [])
+        proto = kind_cases[0]
+        for j in range(n_sub):
+            tag += 1
+            ids = []
+            for _ in range(S):
+                sid = len(slot_case)
+                if mine:
+                    ci = mine.pop(0)
+                    sp = copy.copy(specs[ci])
+                    init = inits[ci]
+                else:
+                    ci = None
+                    sp = copy.copy(specs[proto])
+                    sp.max_steps = 1
+                    init = inits[proto]
+                sp.group = tag
+                slot_case.append(ci)
+                slot_spec.append(sp)
+                slot_init.append(init)
+                ids.append(sid)
+            sub_kind.append(k)
+            sub_slots.append(ids)
+    bs = BatchSolver(slot_spec, scheme="SRK2", threshold=1.0, check_every=CHECK_EVERY, patience=PATIENCE,
+                     autotune=False, launch=LAUNCH)
     results = [None] * n
     try:
-        for j, i in enumerate(first):
-            bs.set_state(j, inits[i])
+        for j, init in enumerate(slot_init):
+            bs.set_state(j, init)
         bs.prepare()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        slot_case = list(first)
+        # retire the placeholders (one step) and tune the launch of the
+        # sub-groups in flight
+        step = bs.run(1)
+        bs.compact()
+        if AUTOTUNE and LAUNCH is None:
+            bs.tune()
         hist = {ci: [] for ci in first}
         restarted = []
-        drained = False
-        step = 0
-        while any(ci is not None for ci in slot_case):
-            step = bs.run(step + chunk)
+        changed = False
+        while any(ci is not None for ci in slot_case) or any(queues.values()):
+            # runs end on multiples of `chunk` (a multiple of the check
+            # period), so replaced cases keep the check cadence of a solo run
+            step = bs.run((step // chunk + 1) * chunk)
             reps = bs.reports()
             for j, ci in enumerate(slot_case):
                 if ci is None:
@@ -284,10 +357,34 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     hist[nxt] = []
                 else:
                     slot_case[j] = None
-                    drained = True
-            if drained:
-                bs.compact()        # groups with no case left stop launching
-                drained = False
+                    changed = True
+            # open idle sub-groups of the kinds with the most waiting cases
+            # (a live sub-group costs all of its slots in every launch, so the
+            # launched slots are bounded as well as the cases in flight)
+            active = sum(ci is not None for ci in slot_case)
+            lanes = S * sum(any(slot_case[j] is not None for j in sl) for sl in sub_slots)
+            while active <= slots - S and lanes + S <= LANE_FACTOR * slots:
+                waiting = [(len(queues[sub_kind[g]]), g) for g in range(len(sub_kind))
+                           if queues[sub_kind[g]] and all(slot_case[j] is None for j in sub_slots[g])]
+                if not waiting:
+                    break
+                g = max(waiting, key=lambda x: (x[0], -x[1]))[1]
+                q = queues[sub_kind[g]]
+                for j in sub_slots[g]:
+                    if not q:
+                        break
+                    nxt = q.popleft()
+                    bs.replace_case(j, specs[nxt], inits[nxt])
+                    slot_case[j] = nxt
+                    hist[nxt] = []
+                    active += 1
+                lanes += S
+                changed = True
+            if changed:
+                bs.compact()        # sub-groups with no case left stop launching
+                changed = False
+            if trace is not None:
+                trace.append((time.perf_counter() - t0, step, active, sum(len(q) for q in queues.values())))
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
     finally:

@@ -30,6 +30,7 @@ class CaseSpec:
     threshold: float = 0.0         # > 0 overrides the solver's stationarity threshold
     tag: dict = field(default_factory=dict)
     max_steps: int = 0             # > 0: per-case step cap (N = min(N, max_steps))
+    group: int = 0                 # cases of different groups never share a launch group
 
 
 def transonic_cfl(mach: float, cfl: float) -> float:
@@ -86,6 +87,7 @@ class BatchSolver:
         c.q_ref = float(cs.q_ref)
         c.threshold = float(cs.threshold)
         c.max_steps = int(cs.max_steps)
+        c.group = int(cs.group)
         if cs.engineering is not None:
             tcd, tcl, cdt, clt, pat = cs.engineering
             c.track_engineering = 1
@@ -196,8 +198,9 @@ class BatchSolver:
         return tuple(x.value for x in v)
 
     def time_kernels(self, reps=20):
-        """Average ms per launch of the (reconstruction, face flux, gather +
-        update, cell-centric flux + update) kernels, HIP events on the current
+        """Average ms per launch set of the (reconstruction, face flux,
+        gather + update, cell-centric flux + update) kernels, timed with HIP
+        events between the kernels of real stage sequences on the current
         stream."""
         ms = (C.c_double * 4)()
         L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), ms, stream_ptr()), "esr_solver_time_kernels")

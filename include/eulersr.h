@@ -146,6 +146,9 @@ typedef struct {
   int64_t max_steps;         /* > 0: N = min(int(tf/dt) + 1, max_steps), a per-case
                                 step cap of the campaign harness (no reference
                                 counterpart); 0: none */
+  int32_t group;             /* cases with different values never share a launch
+                                group (a scheduler's sub-groups of one mesh and
+                                kernel configuration; no reference counterpart); 0 */
 } esr_case;
 
 typedef struct {
@@ -262,11 +265,12 @@ esr_status esr_warm_apply(int64_t rows, int32_t n_hr, const float* idw, const fl
  * cells, face work-items (logical face x case) and distinct group faces. */
 esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* group_cells, int64_t* face_items,
                            int64_t* group_faces);
-/* Average device time (ms) of one launch of each stage kernel over the
+/* Average device time (ms) of one launch set of each stage kernel over the
  * solver's current states: ms[0] reconstruction, ms[1] face flux, ms[2]
  * gather + stage-1 update (the per-face path), ms[3] cell-centric flux +
- * stage-1 update (the cell path). Each is launched `reps` times back to back
- * between two HIP events on `stream`; all are idempotent (they rewrite
+ * stage-1 update (the cell path). Each is timed between HIP events on
+ * `stream` inside the stage sequence it belongs to (recon, face, update;
+ * recon, cell), `reps` sequences each; all are idempotent (they rewrite
  * scratch buffers and the stage buffer only). ms holds 4 entries. */
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
 

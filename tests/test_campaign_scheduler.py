@@ -1,0 +1,166 @@
+"""The campaign scheduler (pipeline.campaign) against a mock solver on the
+CPU: every case is solved exactly once in a slot of its own kind, slots are
+refilled in place, sub-groups keep their kind and launch-group tag, idle
+sub-groups open for the kinds with waiting cases so the cases in flight stay
+near the slot budget, stalled warm starts restart from the fallback state,
+and runs end on multiples of the chunk."""
+import copy
+from dataclasses import dataclass, field
+
+import pytest
+
+from eulersr_amd import pipeline as P
+
+
+@dataclass
+class Spec:
+    mesh: object
+    kw: dict
+    cost: int                       # steps to converge from `inits`
+    max_steps: int = 0
+    group: int = 0
+    tag: dict = field(default_factory=dict)
+
+
+class MockSolver:
+    """Per-slot step counters; a case converges after its cost in steps (a
+    placeholder with max_steps ends after max_steps)."""
+
+    instances = []
+
+    def __init__(self, specs, **kw):
+        self.specs = [copy.copy(s) for s in specs]
+        self.start = [0] * len(specs)
+        self.init = [None] * len(specs)
+        self.step = 0
+        self.log = []
+        self.compacts = 0
+        MockSolver.instances.append(self)
+
+    def set_state(self, j, W):
+        self.init[j] = W
+
+    def prepare(self):
+        pass
+
+    def tune(self):
+        self.log.append(("tune", self.step))
+
+    def _cost(self, j):
+        sp = self.specs[j]
+        c = sp.cost if self.init[j] != "fallback" else sp.tag.get("cold", sp.cost)
+        if self.init[j] == "stuck":
+            c = 10 ** 9
+        return c
+
+    def _report(self, j):
+        sp = self.specs[j]
+        ran = self.step - self.start[j]
+        N = sp.max_steps if sp.max_steps else 10 ** 9
+        conv = ran >= self._cost(j) and ran < N
+        hist = 1e-3 if self.init[j] == "stuck" else 1e-3 / (1 + ran)
+        return dict(converged=bool(ran >= self._cost(j)), diverged=False, steps_run=min(ran, N), N=N,
+                    stopping_step=self._cost(j) if conv else N, last_rel=hist, kind=P._kind(sp), group=sp.group,
+                    case=sp.tag.get("id"))
+
+    def run(self, until):
+        assert until > self.step
+        self.log.append(("run", self.step, until))
+        self.step = until
+        return self.step
+
+    def reports(self):
+        return [self._report(j) for j in range(len(self.specs))]
+
+    def replace_case(self, j, spec, W):
+        assert P._kind(spec) == P._kind(self.specs[j]), "a slot only takes cases of its own kind"
+        g = self.specs[j].group
+        self.specs[j] = copy.copy(spec)
+        self.specs[j].group = g
+        self.start[j] = self.step
+        self.init[j] = W
+        self.log.append(("replace", j, spec.tag.get("id"), self.step))
+
+    def compact(self):
+        self.compacts += 1
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def mock(monkeypatch):
+    import torch
+    monkeypatch.setattr(torch.cuda, "synchronize", lambda *a, **k: None)
+    MockSolver.instances = []
+    monkeypatch.setattr(P, "BatchSolver", MockSolver)
+    monkeypatch.setattr(P, "AUTOTUNE", True)
+    monkeypatch.setattr(P, "LAUNCH", None)
+    return MockSolver
+
+
+def _cases(kinds, per_kind, costs):
+    meshes = [object() for _ in range(kinds)]
+    specs, inits = [], []
+    for i in range(kinds * per_kind):
+        k = i % kinds
+        sp = Spec(meshes[k], dict(flux="HLLC", reconstruction="MUSCL"), cost=costs(i, k))
+        sp.tag = {"id": i, "cold": 2 * costs(i, k)}
+        specs.append(sp)
+        inits.append(f"init{i}")
+    return specs, inits
+
+
+def test_every_case_solved_once_in_its_kind(mock):
+    specs, inits = _cases(3, 20, lambda i, k: 300 + 700 * k + 37 * (i % 5))
+    reps, t, restarted = P.campaign(specs, inits, slots=12, chunk=100)
+    assert restarted == []
+    assert all(r is not None and r["converged"] for r in reps)
+    for i, r in enumerate(reps):
+        assert r["case"] == i and r["kind"] == P._kind(specs[i])
+        assert r["stopping_step"] == specs[i].cost
+    bs = mock.instances[0]
+    # sub-groups: S slots of one kind and one launch-group tag
+    S = P.subgroup_size(specs, 12)
+    assert S == 4
+    tags = {}
+    for j, sp in enumerate(bs.specs):
+        tags.setdefault(sp.group, set()).add(P._kind(sp))
+    assert all(len(v) == 1 for v in tags.values())
+    assert all(sum(1 for sp in bs.specs if sp.group == g) == S for g in tags)
+    # runs end on chunk multiples after the one-step placeholder run
+    runs = [x for x in bs.log if x[0] == "run"]
+    assert runs[0][1:] == (0, 1)
+    assert all(u % 100 == 0 for _, _, u in runs[1:])
+    assert ("tune", 1) in bs.log
+
+
+def test_fast_kinds_hand_their_slots_to_slow_kinds(mock):
+    # kind 0 converges in 100 steps, kind 1 in 2000: once kind 0 has no
+    # waiting case, its share goes to kind 1 and the slots stay busy
+    specs, inits = _cases(2, 40, lambda i, k: 100 if k == 0 else 2000)
+    reps, t, _ = P.campaign(specs, inits, slots=8, chunk=100)
+    assert all(r["converged"] for r in reps)
+    bs = mock.instances[0]
+    # kind 1 starts with half of the 8 slots and ends up using more of them
+    slot_kind1 = {x[1] for x in bs.log if x[0] == "replace" and specs[x[2]].cost == 2000}
+    assert len(slot_kind1) > 4
+    # 40 kind-1 cases x 2000 steps over about 8 slots; the kind-0 work is small
+    total_steps = [x for x in bs.log if x[0] == "run"][-1][2]
+    assert total_steps <= 40 * 2000 / 8 + 2 * 2000
+
+
+def test_stalled_warm_start_restarts_from_fallback(mock):
+    specs, inits = _cases(1, 6, lambda i, k: 1500)
+    inits[2] = "stuck"
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=3, chunk=100, fallback_inits=fb, stall=(1000, 500, 0.97))
+    assert restarted == [2]
+    assert all(r["converged"] for r in reps)
+    assert reps[2]["stopping_step"] == specs[2].tag["cold"]
+
+
+def test_chunk_must_follow_the_check_period(mock):
+    specs, inits = _cases(1, 2, lambda i, k: 100)
+    with pytest.raises(ValueError):
+        P.campaign(specs, inits, slots=2, chunk=42)
