@@ -41,6 +41,8 @@ Phys make_phys(double gamma, double M) {
   p.k_exp = (float)(1.0 / (1.0 - gamma));
   p.inv_gm1M2 = (float)(1.0 / ((gamma - 1.0) * (M * M)));
   p.inv_gm1 = (float)(1.0 / (gamma - 1.0));
+  p.inv_g = (float)(1.0 / gamma);
+  p.inv_2g = (float)(1.0 / (2.0 * gamma));
   return p;
 }
 

@@ -87,6 +87,8 @@ struct Phys {
   float k_exp;      // 1 / (1 - gamma)
   float inv_gm1M2;  // 1 / ((gamma - 1) M**2)
   float inv_gm1;    // 1 / (gamma - 1)
+  float inv_g;      // 1 / gamma
+  float inv_2g;     // 1 / (2 gamma)
 };
 
 // A group: C cases sharing one mesh and one kernel configuration. The state

@@ -417,18 +417,31 @@ __device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR
   F[3] = rh * un * Hh;
 }
 
-__device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
-                                         float D[4]) {
-  // Roe average with clamps (helper.py:101-127)
-  float rL = WL.x;
-  rL = (rL <= 0.0f) ? 1e-6f : rL;
-  const float irL = frcp(rL), uL = WL.y * irL, vL = WL.z * irL;
-  float PL = (WL.w - 0.5f * rL * (uL * uL + vL * vL)) * ph.gm1;
+// Roe dissipation 0.5 R |Lambda S| R^T dv (euler_solver.py:145-199) from the
+// face states' conserved W and primitive q forms.
+__device__ __forceinline__ void diss_roe(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                         float ny, const Phys& ph, float D[4]) {
+  // Roe average (helper.py:101-127): pressure without the M^2 factor of the
+  // primitives, P = q.P / M^2; rho <= 0 is clamped to 1e-6 (then u, v, P
+  // come from the conserved state), P <= 0 to 1e-6
+  const float i0 = frcp(L.r), i1 = frcp(R.r);
+  float rL = L.r, uL = L.u, vL = L.v, PL = L.P * ph.invM2, irL = i0;
+  float rR = R.r, uR = R.u, vR = R.v, PR = R.P * ph.invM2, irR = i1;
+  if (!(rL > 0.0f)) {
+    rL = (rL <= 0.0f) ? 1e-6f : rL;
+    irL = frcp(rL);
+    uL = WL.y * irL;
+    vL = WL.z * irL;
+    PL = (WL.w - 0.5f * rL * (uL * uL + vL * vL)) * ph.gm1;
+  }
+  if (!(rR > 0.0f)) {
+    rR = (rR <= 0.0f) ? 1e-6f : rR;
+    irR = frcp(rR);
+    uR = WR.y * irR;
+    vR = WR.z * irR;
+    PR = (WR.w - 0.5f * rR * (uR * uR + vR * vR)) * ph.gm1;
+  }
   PL = (PL <= 0.0f) ? 1e-6f : PL;
-  float rR = WR.x;
-  rR = (rR <= 0.0f) ? 1e-6f : rR;
-  const float irR = frcp(rR), uR = WR.y * irR, vR = WR.z * irR;
-  float PR = (WR.w - 0.5f * rR * (uR * uR + vR * vR)) * ph.gm1;
   PR = (PR <= 0.0f) ? 1e-6f : PR;
   const float sL = __builtin_sqrtf(rL), sR = __builtin_sqrtf(rR);
   const float isum = frcp(sL + sR);
@@ -459,24 +472,21 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const float4 WR, float
   Rm[3][3] = H + a * (u * nx + v * ny);
   const float un = u * nx + v * ny;
   const float lam[4] = {un - a, un, un, un + a};
-  const float ig = frcp(ph.g), i2g = frcp(ph.two_g);
-  const float S[4] = {r * i2g, ph.gm1 * r * ig, r * ig, r * i2g};
+  const float S[4] = {r * ph.inv_2g, ph.gm1 * r * ph.inv_g, r * ph.inv_g, r * ph.inv_2g};
   // difference of the entropy variables V(W_R) - V(W_L) (helper.py:58-69)
-  // from the unclamped states: V = [(g - s)/(g - 1) - rho |u|^2 / (2P),
+  // of the unclamped states: V = [(g - s)/(g - 1) - rho |u|^2 / (2P),
   // rho u / P, rho v / P, -rho / P], s = ln(P / rho^g); the entropy jump
   // taken as ln(P_R/P_L) - g ln(rho_R/rho_L) (two logarithms, no power)
   float dv[4];
   {
-    const float r0 = WL.x, r1 = WR.x;
-    const float i0 = frcp(r0), i1 = frcp(r1);
-    const float q0 = WL.y * WL.y + WL.z * WL.z, q1 = WR.y * WR.y + WR.z * WR.z;  // rho^2 |u|^2
-    const float P0 = (WL.w - 0.5f * q0 * i0) * ph.gm1, P1 = (WR.w - 0.5f * q1 * i1) * ph.gm1;
+    const float P0 = L.P * ph.invM2, P1 = R.P * ph.invM2;
     const float iP0 = frcp(P0), iP1 = frcp(P1);
-    const float ds = logf(P1 * iP0) - ph.g * logf(r1 * i0);
-    dv[0] = -ds * ph.inv_gm1 - 0.5f * (q1 * i1 * iP1 - q0 * i0 * iP0);
+    const float ds = logf(P1 * iP0) - ph.g * logf(R.r * i0);
+    const float k0 = L.r * (L.u * L.u + L.v * L.v), k1 = R.r * (R.u * R.u + R.v * R.v);  // rho |u|^2
+    dv[0] = -ds * ph.inv_gm1 - 0.5f * (k1 * iP1 - k0 * iP0);
     dv[1] = WR.y * iP1 - WL.y * iP0;
     dv[2] = WR.z * iP1 - WL.z * iP0;
-    dv[3] = r0 * iP0 - r1 * iP1;
+    dv[3] = L.r * iP0 - R.r * iP1;
   }
   float DRH[4];
 #pragma unroll
@@ -530,11 +540,12 @@ __device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const 
   for (int v = 0; v < 4; ++v) D[v] = 0.5f * (FL[v] + FR[v]) - FA[v];
 }
 
+// Flux of one face from its left / right states in conserved (W) and
+// primitive (q) form (the same state; the primitives are the reconstructed
+// values themselves, not a conversion of W back).
 template <int FLUX, int ENT>
-__device__ __forceinline__ void face_flux(const float4 WL, const float4 WR, float nx, float ny, const Phys& ph,
-                                          float lf_c, float F[4]) {
-  const Prim L = prim_of(WL, ph.M2, ph.gm1);
-  const Prim R = prim_of(WR, ph.M2, ph.gm1);
+__device__ __forceinline__ void face_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float lf_c, float F[4]) {
   float Fc[4], D[4];
   constexpr bool IR = (FLUX == ESR_FLUX_TADMOR) || (ENT != 0);
   if (IR)
@@ -545,7 +556,7 @@ __device__ __forceinline__ void face_flux(const float4 WL, const float4 WR, floa
   if (FLUX == ESR_FLUX_HLLC) {
     diss_hllc(WL, L, WR, R, nx, ny, ph, D);
   } else if (FLUX == ESR_FLUX_ROE) {
-    diss_roe(WL, WR, nx, ny, ph, D);
+    diss_roe(WL, L, WR, R, nx, ny, ph, D);
   } else if (FLUX == ESR_FLUX_AUSM) {
     diss_ausm(WL, L, WR, R, nx, ny, ph, D);
   } else if (FLUX == ESR_FLUX_LF) {
@@ -582,7 +593,7 @@ __device__ __forceinline__ void face_flux(const float4 WL, const float4 WR, floa
 template <int RECON>
 __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int32_t c, const CaseDev& cs,
                                             const float4* __restrict__ Ws, const float4* __restrict__ rec,
-                                            float4& WL, float4& WR) {
+                                            float4& WL, Prim& L, float4& WR, Prim& R) {
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
   const uint32_t own = m.f_own[face];
@@ -590,17 +601,19 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
   const int32_t i = (int32_t)(own & FREF_IDX);
   const int k = (int)(own >> 30);
   if (RECON == ESR_RECON_MUSCL) {
-    WL = cons_of(as_prim(rec[ridx_of(G, k, i, c)]), ph.inv_gm1M2);
+    L = as_prim(rec[ridx_of(G, k, i, c)]);
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
       const float4 Wi = Ws[sidx_of(G, i, c)];
       const float2 nk = (kind == G_WALL) ? m.f_nrm[face] : make_float2(0.f, 0.f);
       const float4 Wg = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws,
                                     sidx_of(G, (int32_t)(f & FREF_BIDX), c));
-      WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.inv_gm1M2);
+      R = prim_of(Wg, ph.M2, ph.gm1);
     } else {
-      WR = cons_of(as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)]), ph.inv_gm1M2);
+      R = as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)]);
     }
+    WL = cons_of(L, ph.inv_gm1M2);
+    WR = cons_of(R, ph.inv_gm1M2);
   } else {
     WL = Ws[sidx_of(G, i, c)];
     if (fref_is_bnd(f)) {
@@ -610,6 +623,8 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
     } else {
       WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), c)];
     }
+    L = prim_of(WL, ph.M2, ph.gm1);
+    R = prim_of(WR, ph.M2, ph.gm1);
   }
 }
 
@@ -620,15 +635,16 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
 // face_flux of a launch class: FLUX_HLLC_ROE picks HLLC or Roe by the
 // group's flux (uniform over a block)
 template <int FLUX, int ENT>
-__device__ __forceinline__ void class_flux(int32_t gflux, const float4 WL, const float4 WR, float nx, float ny,
-                                           const Phys& ph, float lf_c, float F[4]) {
+__device__ __forceinline__ void class_flux(int32_t gflux, const float4 WL, const Prim& L, const float4 WR,
+                                           const Prim& R, float nx, float ny, const Phys& ph, float lf_c,
+                                           float F[4]) {
   if (FLUX == FLUX_HLLC_ROE) {
     if (gflux == ESR_FLUX_ROE)
-      face_flux<ESR_FLUX_ROE, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+      face_flux<ESR_FLUX_ROE, ENT>(WL, L, WR, R, nx, ny, ph, lf_c, F);
     else
-      face_flux<ESR_FLUX_HLLC, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+      face_flux<ESR_FLUX_HLLC, ENT>(WL, L, WR, R, nx, ny, ph, lf_c, F);
   } else {
-    face_flux<FLUX, ENT>(WL, WR, nx, ny, ph, lf_c, F);
+    face_flux<FLUX, ENT>(WL, L, WR, R, nx, ny, ph, lf_c, F);
   }
 }
 
@@ -643,12 +659,13 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
   const MeshDev& m = G->m;
   const CaseDev& cs = cases[G->case0 + it.c];
   float4 WL, WR;
-  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, WR);
+  Prim L, R;
+  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, L, WR, R);
   const float2 nk = m.f_nrm[it.idx];
   const float sk = m.f_len[it.idx];
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   float F[4];
-  class_flux<FLUX, ENT>(G->flux, WL, WR, nk.x, nk.y, G->ph, lf_c, F);
+  class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, G->ph, lf_c, F);
   fx[fidx_of(G, it.idx, it.c)] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
 }
 
@@ -664,9 +681,9 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
   const Phys& ph = G->ph;
   CaseDev& cs = cases[G->case0 + it.c];
   float4 WL, WR;
-  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, WR);
+  Prim L, R;
+  face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, L, WR, R);
   const float2 nk = G->m.f_nrm[it.idx];
-  const Prim L = prim_of(WL, ph.M2, ph.gm1), R = prim_of(WR, ph.M2, ph.gm1);
   const float cl = wave_speed(L, nk.x, nk.y, ph), cr = wave_speed(R, nk.x, nk.y, ph);
   const float cm = (cl != cl) ? cl : ((cr != cr) ? cr : fmaxf(cl, cr));
   atomicMax(&cs.lf_cmax, __float_as_uint(cm));
@@ -774,6 +791,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
   const CaseDev& cs = cases[G->case0 + it.c];
   const float4 Wi = Ws[it.sidx];
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
+  // constant reconstruction: the cell's own state on every face
+  const Prim Pi = (RECON == ESR_RECON_MUSCL) ? Prim{} : prim_of(Wi, ph.M2, ph.gm1);
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -781,27 +800,30 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
     const float2 nk = m.nrm[k * n + cell];
     const float sk = m.len[k * n + cell];
     float4 WL, WR;
+    Prim L, R;
+    if (RECON == ESR_RECON_MUSCL) {
+      L = as_prim(rec[ridx_of(G, k, cell, it.c)]);
+      WL = cons_of(L, ph.inv_gm1M2);
+    } else {
+      L = Pi;
+      WL = Wi;
+    }
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
       const float2 nw = (kind == G_WALL) ? nk : make_float2(0.f, 0.f);
       const float4 Wg = ghost_state(kind, Wi, nw, cs.value, m.inlet_subsonic, Ws,
                                     sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
-      if (RECON == ESR_RECON_MUSCL) {
-        WL = cons_of(as_prim(rec[ridx_of(G, k, cell, it.c)]), ph.inv_gm1M2);
-        WR = cons_of(prim_of(Wg, ph.M2, ph.gm1), ph.inv_gm1M2);
-      } else {
-        WL = Wi;
-        WR = Wg;
-      }
+      R = prim_of(Wg, ph.M2, ph.gm1);
+      WR = (RECON == ESR_RECON_MUSCL) ? cons_of(R, ph.inv_gm1M2) : Wg;
     } else if (RECON == ESR_RECON_MUSCL) {
-      WL = cons_of(as_prim(rec[ridx_of(G, k, cell, it.c)]), ph.inv_gm1M2);
-      WR = cons_of(as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)]), ph.inv_gm1M2);
+      R = as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)]);
+      WR = cons_of(R, ph.inv_gm1M2);
     } else {
-      WL = Wi;
       WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+      R = prim_of(WR, ph.M2, ph.gm1);
     }
     float F[4];
-    class_flux<FLUX, ENT>(G->flux, WL, WR, nk.x, nk.y, ph, lf_c, F);
+    class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, ph, lf_c, F);
     Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
   }
   stage_epilogue(mode, G, cs, cell, it.sidx, Wi, Fk, W0, out, acc);
