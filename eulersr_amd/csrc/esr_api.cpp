@@ -1265,8 +1265,11 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
     // Each kernel timed inside the stage sequence it belongs to (face path:
     // recon, face, update; cell path: recon, cell), with an event between
     // kernels, so every launch meets the cache state of a real stage.
+    // events without the system-scope fence of a default event; the time
+    // between two events is the kernel's duration plus the dispatch gap
+    // before it (a profiler's kernel trace counts the duration alone)
     std::vector<hipEvent_t> ev(6 * (size_t)reps);
-    for (auto& x : ev) hip_check(hipEventCreate(&x), "event");
+    for (auto& x : ev) hip_check(hipEventCreateWithFlags(&x, hipEventDisableSystemFence), "event");
     for (int32_t r = 0; r < reps; ++r) {
       hipEvent_t* q = ev.data() + 6 * (size_t)r;
       hip_check(hipEventRecord(q[0], st), "record");

@@ -270,8 +270,9 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
  * gather + stage-1 update (the per-face path), ms[3] cell-centric flux +
  * stage-1 update (the cell path). Each is timed between HIP events on
  * `stream` inside the stage sequence it belongs to (recon, face, update;
- * recon, cell), `reps` sequences each; all are idempotent (they rewrite
- * scratch buffers and the stage buffer only). ms holds 4 entries. */
+ * recon, cell), `reps` sequences each, so a time includes the dispatch gap
+ * before the kernel; all are idempotent (they rewrite scratch buffers and
+ * the stage buffer only). ms holds 4 entries. */
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
 
 /* Stage path of the engines built after the call (solvers created, and the
