@@ -267,8 +267,10 @@ __device__ __forceinline__ float sound(const Prim& q, const Phys& ph) {
   return __builtin_sqrtf(fabsf(ph.g * q.P * frcp(q.r))) * ph.invM;
 }
 
-__device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
-                                          float ny, const Phys& ph, float D[4]) {
+// HLLC flux F_HLLC (euler_solver.py:273-372); FL, FR receive the physical
+// fluxes of the two states.
+__device__ __forceinline__ void hllc_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float F[4], float FL[4], float FR[4]) {
   const float unL = L.u * nx + L.v * ny;
   const float unR = R.u * nx + R.v * ny;
   const float tx = -ny, ty = nx;
@@ -280,13 +282,11 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
   float SR = fmaxf(unL + aL, unR + aR);
   SL = (SL == SR) ? SL - 1e-6f : SL;
   SR = (SL == SR) ? SR + 1e-6f : SR;
-  float FL[4], FR[4];
   phys_flux(WL, L, unL, nx, ny, ph.invM2, FL);
   phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
   float den = L.r * (SL - unL) - R.r * (SR - unR);
   den = (den == 0.0f) ? 1e-12f : den;
   const float Ss = fdiv(R.P - L.P + L.r * unL * (SL - unL) - R.r * unR * (SR - unR), den);
-  float F[4];
   if (SL >= 0.0f) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) F[v] = FL[v];
@@ -317,6 +317,13 @@ __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const 
       for (int v = 0; v < 4; ++v) F[v] = FR[v];
     }
   }
+}
+
+// HLLC dissipation D = central - F_HLLC (euler_solver.py:373-374)
+__device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float D[4]) {
+  float F[4], FL[4], FR[4];
+  hllc_flux(WL, L, WR, R, nx, ny, ph, F, FL, FR);
 #pragma unroll
   for (int v = 0; v < 4; ++v) D[v] = 0.5f * (FL[v] + FR[v]) - F[v];
 }
@@ -511,8 +518,9 @@ __device__ __forceinline__ void ausm_split(float Mn, float& Mp, float& Mm, float
   Pm = sub ? 0.25f * ((Mn - 1.0f) * (Mn - 1.0f)) * (2.0f + Mn) - alpha * Mn * q : 0.5f * (1.0f - sg);
 }
 
-__device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
-                                          float ny, const Phys& ph, float D[4]) {
+// AUSM+ flux F_AUSM (euler_solver.py:490-529)
+__device__ __forceinline__ void ausm_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float FA[4]) {
   const float unL = L.u * nx + L.v * ny;
   const float unR = R.u * nx + R.v * ny;
   const float aL = sound(L, ph);
@@ -528,12 +536,19 @@ __device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const 
   const float pf = PpL * L.P + PmR * R.P;
   const float HL = fdiv(WL.w + L.P * ph.invM2, L.r);
   const float HR = fdiv(WR.w + R.P * ph.invM2, R.r);
-  float FA[4];
   FA[0] = mL + mR;
   FA[1] = mL * L.u + mR * R.u + pf * nx * ph.invM2;
   FA[2] = mL * L.v + mR * R.v + pf * ny * ph.invM2;
   FA[3] = mL * HL + mR * HR;
-  float FL[4], FR[4];
+}
+
+// AUSM+ dissipation D = central - F_AUSM (euler_solver.py:530)
+__device__ __forceinline__ void diss_ausm(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
+                                          float ny, const Phys& ph, float D[4]) {
+  float FA[4], FL[4], FR[4];
+  ausm_flux(WL, L, WR, R, nx, ny, ph, FA);
+  const float unL = L.u * nx + L.v * ny;
+  const float unR = R.u * nx + R.v * ny;
   phys_flux(WL, L, unL, nx, ny, ph.invM2, FL);
   phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
 #pragma unroll
@@ -548,6 +563,17 @@ __device__ __forceinline__ void face_flux(const float4 WL, const Prim& L, const 
                                           float ny, const Phys& ph, float lf_c, float F[4]) {
   float Fc[4], D[4];
   constexpr bool IR = (FLUX == ESR_FLUX_TADMOR) || (ENT != 0);
+  // HLLC and AUSM+ define their dissipation as central - F_scheme, so the
+  // total flux central - D is the scheme's own flux (to rounding)
+  if (!IR && FLUX == ESR_FLUX_HLLC) {
+    float FL[4], FR[4];
+    hllc_flux(WL, L, WR, R, nx, ny, ph, F, FL, FR);
+    return;
+  }
+  if (!IR && FLUX == ESR_FLUX_AUSM) {
+    ausm_flux(WL, L, WR, R, nx, ny, ph, F);
+    return;
+  }
   if (IR)
     ismail_roe_flux(WL, WR, nx, ny, ph, Fc);
   else
