@@ -34,8 +34,8 @@ starts, the stage kernels' HBM rates at the campaign's 64-case launch, and the
 reference's single-case configuration (6 cases: diamond + NACA0012, M in
 {0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
 for the slowest stage kernel of the autotuned launch of the campaign's
-64-case batch, timed with HIP events on the solver's stream between the
-kernels of real stage sequences; its
+64-case batch (from their warm-start states), timed with HIP events on the
+solver's stream between the kernels of real stage sequences; its
 algorithmic bytes per launch come from the launch's work-items
 (``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
 
@@ -184,16 +184,16 @@ def make_workload(meshes, grid, cap_ref=0):
     return cases, ref, t_ref
 
 
-def stage_kernels(cases, reps, launch=None):
+def stage_kernels(cases, reps, launch=None, inits=None):
     """Live HIP-event timing of the stage kernels on a batched solver of the
-    given HR cases (freestream states advanced 10 steps), autotuned or with a
-    pinned launch configuration."""
+    given HR cases (from `inits`, default freestream, advanced 10 steps),
+    autotuned or with a pinned launch configuration."""
     from eulersr_amd.solve import BatchSolver
     bs = BatchSolver([c.hr_spec for c in cases], scheme="SRK2", threshold=0.0, check_every=5, patience=3,
                      autotune=launch is None, launch=launch)
     try:
         for i, c in enumerate(cases):
-            bs.set_state(i, c.hr_init)
+            bs.set_state(i, c.hr_init if inits is None else inits[i])
         bs.prepare()
         bs.run(10)
         sizes = bs.info()
@@ -441,9 +441,12 @@ def main():
         log(f"cold-start campaign: {cold['total_s']:.2f}s -> {rate(cold):.0f} cases/hour, steps median "
             f"{np.median(cst) if cst else 0:.0f} mean {np.mean(cst) if cst else 0:.0f}")
 
-    # the stage kernels at the campaign's launch: its first `slots` cases
+    # the stage kernels at the campaign's launch: its first `slots` cases,
+    # from their warm-start states
     _, first = P.initial_slots([c.hr_spec for c in cases], args.slots)
-    kern, sizes, tuning = stage_kernels([cases[i] for i in first], args.kernel_reps, P.LAUNCH)
+    first_cases = [cases[i] for i in first]
+    warm_inits, _, _, _ = starter.initial_states(first_cases, "mlp")
+    kern, sizes, tuning = stage_kernels(first_cases, args.kernel_reps, P.LAUNCH, inits=warm_inits)
     log(f"autotuned launch of the {len(first)}-slot batch: {tuning}")
     for name, k in kern.items():
         log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.1f} MB -> "
