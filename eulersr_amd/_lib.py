@@ -133,11 +133,19 @@ _lib = None
 
 
 def lib():
-    """Load the in-tree library (raises if it has not been built)."""
+    """Load the in-tree library (raises if it has not been built).
+
+    torch is imported first: its wheel bundles a HIP runtime with the same
+    soname (libamdhip64.so.7) as the ROCm install the library links against.
+    Loaded after torch, the library binds to torch's runtime, so the process
+    has one runtime and device pointers and streams from torch are valid in
+    the library. Loaded before torch, the process would end up with two
+    runtimes, and the library's own would not see the GPU."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m eulersr_amd.build`")
+        import torch  # noqa: F401  (one HIP runtime per process, see above)
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

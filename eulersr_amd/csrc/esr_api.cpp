@@ -253,8 +253,10 @@ struct Engine {
         g.cshift = 0;
         g.cmagic = 0;
       }
-      // in-group indices are 32-bit: (3 n + nf) C work-items must stay below 2^31
-      if ((3 * (int64_t)m->n + (int64_t)m->dev.nf) * g.C >= ((int64_t)1 << 31))
+      // in-group offsets are 32-bit byte offsets: every slab of the group
+      // (state n C, reconstruction 3 n C, face flux nf C float4 elements)
+      // must stay below 4 GiB, i.e. (3 n + nf) C < 2^28
+      if ((3 * (int64_t)m->n + (int64_t)m->dev.nf) * g.C >= ((int64_t)1 << 28))
         throw Error(ESR_ERR_VALUE, "too many cases on one mesh with one kernel configuration for one solver");
       g.case0 = case0;
       g.st_off = st;

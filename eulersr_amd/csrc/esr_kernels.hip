@@ -69,8 +69,50 @@ __device__ __forceinline__ Prim as_prim(const float4 p) { return Prim{p.x, p.y, 
 
 // Boundary ghost state, helper.py:173-204 (walls and the subsonic inlet use
 // the default gamma = 1.4 and M = 1 of getPrimitive/getConserved).
+// Loads and stores at a 32-bit element offset from a wave-uniform base (a
+// group's slab or a mesh array) in global memory (every buffer here comes
+// from hipMalloc): the access is a global one (not flat, which a pointer read
+// from a struct would otherwise get) addressed as a scalar base plus a
+// 32-bit vector byte offset, with no 64-bit address arithmetic. Every slab of
+// a group and every mesh array stays below 4 GiB (checked when groups are
+// built).
+typedef __attribute__((address_space(1))) const char gconst_char;
+typedef __attribute__((address_space(1))) char gchar;
+// the access goes through a builtin type (HIP's float4 / float2 are classes
+// whose copy operations take generic references)
+typedef float raw_float4 __attribute__((ext_vector_type(4)));
+typedef float raw_float2 __attribute__((ext_vector_type(2)));
+template <class T>
+struct Raw {
+  typedef T type;
+  static __device__ __forceinline__ T of(const T v) { return v; }
+  static __device__ __forceinline__ T to(const T v) { return v; }
+};
+template <>
+struct Raw<float4> {
+  typedef raw_float4 type;
+  static __device__ __forceinline__ float4 of(const raw_float4 v) { return make_float4(v.x, v.y, v.z, v.w); }
+  static __device__ __forceinline__ raw_float4 to(const float4 v) { return raw_float4{v.x, v.y, v.z, v.w}; }
+};
+template <>
+struct Raw<float2> {
+  typedef raw_float2 type;
+  static __device__ __forceinline__ float2 of(const raw_float2 v) { return make_float2(v.x, v.y); }
+  static __device__ __forceinline__ raw_float2 to(const float2 v) { return raw_float2{v.x, v.y}; }
+};
+template <class T>
+__device__ __forceinline__ T ld(const T* base, uint32_t e) {
+  typedef __attribute__((address_space(1))) const typename Raw<T>::type gR;
+  return Raw<T>::of(*(gR*)((gconst_char*)(const char*)base + e * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, uint32_t e, const T v) {
+  typedef __attribute__((address_space(1))) typename Raw<T>::type gR;
+  *(gR*)((gchar*)(char*)base + e * (uint32_t)sizeof(T)) = Raw<T>::to(v);
+}
+
 __device__ __forceinline__ float4 ghost_state(uint32_t kind, const float4 Wi, const float2 n, const float4 value,
-                                              const float4 insub, const float4* __restrict__ Ws, int64_t wrap_idx) {
+                                              const float4 insub, const float4* __restrict__ Wg, uint32_t wrap_e) {
   if (kind == G_WALL) {
     const Prim p = prim_of(Wi, 1.0f, 0.4f);
     const float vn = p.u * n.x + p.v * n.y;
@@ -85,7 +127,7 @@ __device__ __forceinline__ float4 ghost_state(uint32_t kind, const float4 Wi, co
     const float r = insub.x, u = insub.y, v = insub.z;
     return make_float4(r, r * u, r * v, p.P * 2.5f + 0.5f * r * (u * u + v * v));
   }
-  return Ws[wrap_idx];
+  return ld(Wg, wrap_e);
 }
 
 // ---------------------------------------------------------------------------
@@ -96,7 +138,7 @@ struct Item {
   const GroupDev* G;
   int32_t idx;  // cell (cell kernels) or logical face (face kernels)
   int32_t c;
-  int64_t sidx; // state index (cell kernels)
+  uint32_t e;   // element offset in the group's state slab (cell kernels)
 };
 
 // XCD-aware block order (a tuning choice): the hardware deals workgroup p to
@@ -133,19 +175,24 @@ __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, co
   it.G = G;
   it.idx = idx;
   it.c = c;
-  it.sidx = FACES ? 0 : G->st_off + (int64_t)(int32_t)t;
+  it.e = FACES ? 0u : t;
   return true;
 }
 
-// element offsets inside a group are 32-bit (checked when the groups are built)
-__device__ __forceinline__ int64_t sidx_of(const GroupDev* G, int32_t cell, int32_t c) {
-  return G->st_off + (int64_t)(cell * G->C + c);
+// element offsets inside a group's slabs (state [n][C], reconstruction
+// [3][n][C], face flux [nf][C]); 32-bit, checked when the groups are built
+__device__ __forceinline__ uint32_t sel(const GroupDev* G, int32_t cell, int32_t c) {
+  return (uint32_t)cell * (uint32_t)G->C + (uint32_t)c;
 }
-__device__ __forceinline__ int64_t ridx_of(const GroupDev* G, int k, int32_t cell, int32_t c) {
-  return G->rec_off + (int64_t)((k * G->m.n + cell) * G->C + c);
+__device__ __forceinline__ uint32_t rel(const GroupDev* G, int k, int32_t cell, int32_t c) {
+  return ((uint32_t)k * (uint32_t)G->m.n + (uint32_t)cell) * (uint32_t)G->C + (uint32_t)c;
 }
-__device__ __forceinline__ int64_t fidx_of(const GroupDev* G, int32_t face, int32_t c) {
-  return G->fx_off + (int64_t)(face * G->C + c);
+__device__ __forceinline__ uint32_t fel(const GroupDev* G, int32_t face, int32_t c) {
+  return (uint32_t)face * (uint32_t)G->C + (uint32_t)c;
+}
+// mesh [k][n] arrays
+__device__ __forceinline__ uint32_t kn(int k, int32_t n, int32_t cell) {
+  return (uint32_t)k * (uint32_t)n + (uint32_t)cell;
 }
 
 // ---------------------------------------------------------------------------
@@ -168,34 +215,36 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
   const Phys& ph = G->ph;
   const int32_t n = m.n, cell = it.idx;
   const CaseDev& cs = cases[G->case0 + it.c];
+  const float4* Wg = Ws + G->st_off;
+  float4* rg = rec + G->rec_off;
 
-  const float4 Wi = Ws[it.sidx];
+  const float4 Wi = ld(Wg, it.e);
   const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
 
   Prim pr[3];
   bool bnd[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[k * n + cell];
+    const uint32_t f = ld(m.fref, kn(k, n, cell));
     float4 WR;
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? m.nrm[k * n + cell] : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
+      const float2 nk = (kind == G_WALL) ? ld(m.nrm, kn(k, n, cell)) : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Wg, sel(G, (int32_t)(f & FREF_BIDX), it.c));
       bnd[k] = true;
     } else {
-      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+      WR = ld(Wg, sel(G, (int32_t)(f & FREF_IDX), it.c));
       bnd[k] = false;
     }
     pr[k] = prim_of(WR, ph.M2, ph.gm1);
   }
 
   // weighted least squares (helper.py:136-153): grad = A^-1 sum_k w_k dW_k dl_k
-  const float2 w0 = m.wdl[cell], w1 = m.wdl[(int64_t)n + cell], w2 = m.wdl[2 * (int64_t)n + cell];
-  const float4 ai = m.ainv[cell];
+  const float2 w0 = ld(m.wdl, kn(0, n, cell)), w1 = ld(m.wdl, kn(1, n, cell)), w2 = ld(m.wdl, kn(2, n, cell));
+  const float4 ai = ld(m.ainv, (uint32_t)cell);
   float2 dm[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) dm[k] = m.dxm[k * n + cell];
+  for (int k = 0; k < 3; ++k) dm[k] = ld(m.dxm, kn(k, n, cell));
 
   const float qi[4] = {pi.r, pi.u, pi.v, pi.P};
   float D[3][4];
@@ -234,7 +283,7 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
     o.y = bnd[k] ? qi[1] : qi[1] + phi[1] * D[k][1];
     o.z = bnd[k] ? qi[2] : qi[2] + phi[2] * D[k][2];
     o.w = bnd[k] ? qi[3] : qi[3] + phi[3] * D[k][3];
-    rec[ridx_of(G, k, cell, it.c)] = o;
+    st(rg, rel(G, k, cell, it.c), o);
   }
 }
 
@@ -622,32 +671,34 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
                                             float4& WL, Prim& L, float4& WR, Prim& R) {
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
-  const uint32_t own = m.f_own[face];
-  const uint32_t f = m.f_ref[face];
+  const float4* Wsg = Ws + G->st_off;
+  const float4* rg = rec + G->rec_off;
+  const uint32_t own = ld(m.f_own, (uint32_t)face);
+  const uint32_t f = ld(m.f_ref, (uint32_t)face);
   const int32_t i = (int32_t)(own & FREF_IDX);
   const int k = (int)(own >> 30);
   if (RECON == ESR_RECON_MUSCL) {
-    L = as_prim(rec[ridx_of(G, k, i, c)]);
+    L = as_prim(ld(rg, rel(G, k, i, c)));
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float4 Wi = Ws[sidx_of(G, i, c)];
-      const float2 nk = (kind == G_WALL) ? m.f_nrm[face] : make_float2(0.f, 0.f);
-      const float4 Wg = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Ws,
-                                    sidx_of(G, (int32_t)(f & FREF_BIDX), c));
-      R = prim_of(Wg, ph.M2, ph.gm1);
+      const float4 Wi = ld(Wsg, sel(G, i, c));
+      const float2 nk = (kind == G_WALL) ? ld(m.f_nrm, (uint32_t)face) : make_float2(0.f, 0.f);
+      const float4 Wgh = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Wsg,
+                                     sel(G, (int32_t)(f & FREF_BIDX), c));
+      R = prim_of(Wgh, ph.M2, ph.gm1);
     } else {
-      R = as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)]);
+      R = as_prim(ld(rg, rel(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)));
     }
     WL = cons_of(L, ph.inv_gm1M2);
     WR = cons_of(R, ph.inv_gm1M2);
   } else {
-    WL = Ws[sidx_of(G, i, c)];
+    WL = ld(Wsg, sel(G, i, c));
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? m.f_nrm[face] : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, WL, nk, cs.value, m.inlet_subsonic, Ws, sidx_of(G, (int32_t)(f & FREF_BIDX), c));
+      const float2 nk = (kind == G_WALL) ? ld(m.f_nrm, (uint32_t)face) : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, WL, nk, cs.value, m.inlet_subsonic, Wsg, sel(G, (int32_t)(f & FREF_BIDX), c));
     } else {
-      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), c)];
+      WR = ld(Wsg, sel(G, (int32_t)(f & FREF_IDX), c));
     }
     L = prim_of(WL, ph.M2, ph.gm1);
     R = prim_of(WR, ph.M2, ph.gm1);
@@ -687,12 +738,12 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
   float4 WL, WR;
   Prim L, R;
   face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, L, WR, R);
-  const float2 nk = m.f_nrm[it.idx];
-  const float sk = m.f_len[it.idx];
+  const float2 nk = ld(m.f_nrm, (uint32_t)it.idx);
+  const float sk = ld(m.f_len, (uint32_t)it.idx);
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   float F[4];
   class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, G->ph, lf_c, F);
-  fx[fidx_of(G, it.idx, it.c)] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
+  st(fx + G->fx_off, fel(G, it.idx, it.c), make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]));
 }
 
 // Lax-Friedrichs: per-case maximum wave speed over both states of every face
@@ -709,7 +760,7 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
   float4 WL, WR;
   Prim L, R;
   face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, L, WR, R);
-  const float2 nk = G->m.f_nrm[it.idx];
+  const float2 nk = ld(G->m.f_nrm, (uint32_t)it.idx);
   const float cl = wave_speed(L, nk.x, nk.y, ph), cr = wave_speed(R, nk.x, nk.y, ph);
   const float cm = (cl != cl) ? cl : ((cr != cr) ? cr : fmaxf(cl, cr));
   atomicMax(&cs.lf_cmax, __float_as_uint(cm));
@@ -721,28 +772,30 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
 
 // residual of one cell from its three face fluxes (F * len, outward), the
 // sponge source, and the stage update selected by `mode` (a StageMode)
-__device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell, int64_t s,
+// (s: the cell's element offset in the group slabs W0, out and acc, which
+// are the group's base pointers)
+__device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell, uint32_t s,
                                                const float4 Wi, const float4 Fk[3], const float4* W0, float4* out,
                                                float4* acc) {
   const MeshDev& m = G->m;
-  const float ia = m.inv_area[cell];
+  const float ia = ld(m.inv_area, (uint32_t)cell);
   float R[4];
   R[0] = ((Fk[0].x + Fk[1].x) + Fk[2].x) * ia;
   R[1] = ((Fk[0].y + Fk[1].y) + Fk[2].y) * ia;
   R[2] = ((Fk[0].z + Fk[1].z) + Fk[2].z) * ia;
   R[3] = ((Fk[0].w + Fk[1].w) + Fk[2].w) * ia;
   if (G->sponge) {
-    const float sigma = cs.strength * G->sponge_g[cell];
+    const float sigma = cs.strength * ld(G->sponge_g, (uint32_t)cell);
     R[0] = R[0] - sigma * (cs.value.x - Wi.x);
     R[1] = R[1] - sigma * (cs.value.y - Wi.y);
     R[2] = R[2] - sigma * (cs.value.z - Wi.z);
     R[3] = R[3] - sigma * (cs.value.w - Wi.w);
   }
   if (mode == MODE_RESID) {
-    out[s] = make_float4(R[0], R[1], R[2], R[3]);
+    st(out, s, make_float4(R[0], R[1], R[2], R[3]));
     return;
   }
-  const float4 w0 = (mode == MODE_EE) ? Wi : W0[s];
+  const float4 w0 = (mode == MODE_EE) ? Wi : ld(W0, s);
   float4 o;
   if (mode == MODE_EE || mode == MODE_S1) {
     const float dt = cs.dt;
@@ -758,19 +811,19 @@ __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, cons
   } else if (mode == MODE_RK4_1) {
     const float h = cs.dt_half;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
-    acc[s] = make_float4(R[0], R[1], R[2], R[3]);
+    st(acc, s, make_float4(R[0], R[1], R[2], R[3]));
   } else if (mode == MODE_RK4_2 || mode == MODE_RK4_3) {
     const float h = (mode == MODE_RK4_2) ? cs.dt_half : cs.dt;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
-    const float4 a = acc[s];
-    acc[s] = make_float4(a.x + 2.0f * R[0], a.y + 2.0f * R[1], a.z + 2.0f * R[2], a.w + 2.0f * R[3]);
+    const float4 a = ld(acc, s);
+    st(acc, s, make_float4(a.x + 2.0f * R[0], a.y + 2.0f * R[1], a.z + 2.0f * R[2], a.w + 2.0f * R[3]));
   } else {  // MODE_RK4_4
     const float h = cs.dt_sixth;
-    const float4 a = acc[s];
+    const float4 a = ld(acc, s);
     o = make_float4(w0.x - h * (a.x + R[0]), w0.y - h * (a.y + R[1]), w0.z - h * (a.z + R[2]),
                     w0.w - h * (a.w + R[3]));
   }
-  out[s] = o;
+  st(out, s, o);
 }
 
 template <int MODE>
@@ -785,14 +838,16 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
   const MeshDev& m = G->m;
   const int32_t n = m.n, cell = it.idx;
   const CaseDev& cs = cases[G->case0 + it.c];
+  const float4* fg = fx + G->fx_off;
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t cf = m.cface[k * n + cell];
-    const float4 f = fx[fidx_of(G, (int32_t)(cf & ~CFACE_NEG), it.c)];
+    const uint32_t cf = ld(m.cface, kn(k, n, cell));
+    const float4 f = ld(fg, fel(G, (int32_t)(cf & ~CFACE_NEG), it.c));
     Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
-  stage_epilogue(MODE, G, cs, cell, it.sidx, Ws[it.sidx], Fk, W0, out, acc);
+  const int64_t so = G->st_off;
+  stage_epilogue(MODE, G, cs, cell, it.e, ld(Ws + so, it.e), Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -815,20 +870,22 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
   const Phys& ph = G->ph;
   const int32_t n = m.n, cell = it.idx;
   const CaseDev& cs = cases[G->case0 + it.c];
-  const float4 Wi = Ws[it.sidx];
+  const float4* Wg = Ws + G->st_off;
+  const float4* rg = rec + G->rec_off;
+  const float4 Wi = ld(Wg, it.e);
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   // constant reconstruction: the cell's own state on every face
   const Prim Pi = (RECON == ESR_RECON_MUSCL) ? Prim{} : prim_of(Wi, ph.M2, ph.gm1);
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = m.fref[k * n + cell];
-    const float2 nk = m.nrm[k * n + cell];
-    const float sk = m.len[k * n + cell];
+    const uint32_t f = ld(m.fref, kn(k, n, cell));
+    const float2 nk = ld(m.nrm, kn(k, n, cell));
+    const float sk = ld(m.len, kn(k, n, cell));
     float4 WL, WR;
     Prim L, R;
     if (RECON == ESR_RECON_MUSCL) {
-      L = as_prim(rec[ridx_of(G, k, cell, it.c)]);
+      L = as_prim(ld(rg, rel(G, k, cell, it.c)));
       WL = cons_of(L, ph.inv_gm1M2);
     } else {
       L = Pi;
@@ -837,22 +894,23 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
     if (fref_is_bnd(f)) {
       const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
       const float2 nw = (kind == G_WALL) ? nk : make_float2(0.f, 0.f);
-      const float4 Wg = ghost_state(kind, Wi, nw, cs.value, m.inlet_subsonic, Ws,
-                                    sidx_of(G, (int32_t)(f & FREF_BIDX), it.c));
-      R = prim_of(Wg, ph.M2, ph.gm1);
-      WR = (RECON == ESR_RECON_MUSCL) ? cons_of(R, ph.inv_gm1M2) : Wg;
+      const float4 Wgh = ghost_state(kind, Wi, nw, cs.value, m.inlet_subsonic, Wg,
+                                     sel(G, (int32_t)(f & FREF_BIDX), it.c));
+      R = prim_of(Wgh, ph.M2, ph.gm1);
+      WR = (RECON == ESR_RECON_MUSCL) ? cons_of(R, ph.inv_gm1M2) : Wgh;
     } else if (RECON == ESR_RECON_MUSCL) {
-      R = as_prim(rec[ridx_of(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)]);
+      R = as_prim(ld(rg, rel(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)));
       WR = cons_of(R, ph.inv_gm1M2);
     } else {
-      WR = Ws[sidx_of(G, (int32_t)(f & FREF_IDX), it.c)];
+      WR = ld(Wg, sel(G, (int32_t)(f & FREF_IDX), it.c));
       R = prim_of(WR, ph.M2, ph.gm1);
     }
     float F[4];
     class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, ph, lf_c, F);
     Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
   }
-  stage_epilogue(mode, G, cs, cell, it.sidx, Wi, Fk, W0, out, acc);
+  const int64_t so = G->st_off;
+  stage_epilogue(mode, G, cs, cell, it.e, Wi, Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1238,7 +1296,7 @@ __global__ __launch_bounds__(BLOCK) void k_commit(const GroupDev* __restrict__ g
                                                   float4* __restrict__ dst, int xcd) {
   Item it;
   if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
-  dst[it.sidx] = src[it.sidx];
+  st(dst + it.G->st_off, it.e, ld(src + it.G->st_off, it.e));
 }
 
 // ---------------------------------------------------------------------------

@@ -1,6 +1,5 @@
 """The C-ABI library loads and exports every symbol that include/eulersr.h
 declares; the product path has no CPU fallback."""
-import ctypes
 import re
 from pathlib import Path
 
@@ -25,7 +24,7 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     from eulersr_amd import _lib
-    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    lib = _lib.lib()
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     # the ctypes table covers the header exactly
