@@ -782,13 +782,23 @@ esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_s
   return guard([&] {
     if (!out || !cases || !so || n_cases <= 0) throw Error(ESR_ERR_VALUE, "bad solver arguments");
     if (so->check_every < 1) throw Error(ESR_ERR_VALUE, "check_every must be >= 1");
-    if (so->local_dt) throw Error(ESR_ERR_VALUE, "local time stepping is not available in this build");
     std::unique_ptr<esr_solver> s(new esr_solver());
     s->so = *so;
     s->cases.assign(cases, cases + n_cases);
     std::vector<CaseSpec> specs;
     for (int32_t i = 0; i < n_cases; ++i) specs.push_back(CaseSpec{cases[i].mesh, cases[i].opts, cases[i].group});
     s->eng.build(specs, so->scheme);
+    if (so->local_dt) {
+      // local time stepping: each case's cells advance with their own dt at
+      // the case's CFL (the stage kernels read it from the case parameters)
+      Engine& e = s->eng;
+      for (int32_t u = 0; u < n_cases; ++u) {
+        if (!(cases[u].cfl > 0.f)) throw Error(ESR_ERR_VALUE, "cfl must be > 0");
+        e.cases_h[e.global_of_user[u]].cfl_local = cases[u].cfl;
+      }
+      hip_check(hipMemcpy(e.cases_d.p, e.cases_h.data(), e.cases_h.size() * sizeof(CaseDev),
+                          hipMemcpyHostToDevice), "case params H2D");
+    }
     s->dt.assign(n_cases, 0.f);
     s->N.assign(n_cases, 0);
     s->final_residual.assign(n_cases, NAN);
@@ -1088,6 +1098,7 @@ esr_status esr_solver_replace_case(esr_solver* s, int32_t c, const esr_case* spe
     e.gather_in(c, W, e.W0.as<float4>(), st);
     e.gather_in(c, W, e.Wref.as<float4>(), st);
     CaseDev cd = Engine::case_params(spec->mesh, b);
+    cd.cfl_local = s->so.local_dt ? spec->cfl : 0.f;
     e.cases_h[gc] = cd;
     launch_set_case(e.cases() + gc, cd, st);
     const float dt = e.compute_dt_one(gc, spec->cfl, st);

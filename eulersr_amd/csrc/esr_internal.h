@@ -134,7 +134,7 @@ struct CaseDev {
   uint32_t lf_cmax;      // float bits of the Lax-Friedrichs global wave speed
   int32_t active;        // 0 once stopped or past N
   int32_t check;         // 1 when the current step is a stationarity check
-  int32_t pad;
+  float cfl_local;       // > 0: local time stepping with this CFL (each cell its own dt); 0: global dt
 };
 
 // Per-case solve-loop state (device), main.py:141-199.

@@ -195,6 +195,23 @@ __device__ __forceinline__ uint32_t kn(int k, int32_t n, int32_t cell) {
   return (uint32_t)k * (uint32_t)n + (uint32_t)cell;
 }
 
+// area / sum_k (c + |u . n_k|) len_k of one cell from its state W: the time
+// step of the cell at CFL 1 (helper.py:8-18 before the minimum over cells
+// and the CFL factor). k_dt takes its minimum (the reference's global dt);
+// local time stepping uses it per cell.
+__device__ __forceinline__ float cell_dt1(const MeshDev& m, const Phys& ph, int32_t cell, const float4 W) {
+  const Prim q = prim_of(W, ph.M2, ph.gm1);
+  const float cel = sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM;
+  float tk[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float2 nk = ld(m.nrm, kn(k, m.n, cell));
+    const float un = q.u * nk.x + q.v * nk.y;
+    tk[k] = (cel + fabsf(un)) * ld(m.len, kn(k, m.n, cell));
+  }
+  return ld(m.area, (uint32_t)cell) / ((tk[0] + tk[1]) + tk[2]);
+}
+
 // ---------------------------------------------------------------------------
 // reconstruction: LSQ gradient + Venkatakrishnan + MUSCL face states
 // ---------------------------------------------------------------------------
@@ -796,29 +813,34 @@ __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, cons
     return;
   }
   const float4 w0 = (mode == MODE_EE) ? Wi : ld(W0, s);
+  // time step: the case's global dt, or with local time stepping the cell's
+  // own dt from the step's start state w0 (the same dt in every stage)
+  float dt = cs.dt, dt_half = cs.dt_half, dt_sixth = cs.dt_sixth;
+  if (cs.cfl_local > 0.0f) {
+    dt = cell_dt1(G->m, G->ph, cell, w0) * cs.cfl_local;
+    dt_half = dt / 2.0f;
+    dt_sixth = dt / 6.0f;
+  }
   float4 o;
   if (mode == MODE_EE || mode == MODE_S1) {
-    const float dt = cs.dt;
     o = make_float4(w0.x - dt * R[0], w0.y - dt * R[1], w0.z - dt * R[2], w0.w - dt * R[3]);
   } else if (mode == MODE_SRK2_2) {
-    const float dt = cs.dt;
     o = make_float4(0.5f * ((w0.x + Wi.x) - dt * R[0]), 0.5f * ((w0.y + Wi.y) - dt * R[1]),
                     0.5f * ((w0.z + Wi.z) - dt * R[2]), 0.5f * ((w0.w + Wi.w) - dt * R[3]));
   } else if (mode == MODE_RK2_2) {
-    const float dt = cs.dt;
     o = make_float4(0.5f * ((w0.x - dt * R[0]) + Wi.x), 0.5f * ((w0.y - dt * R[1]) + Wi.y),
                     0.5f * ((w0.z - dt * R[2]) + Wi.z), 0.5f * ((w0.w - dt * R[3]) + Wi.w));
   } else if (mode == MODE_RK4_1) {
-    const float h = cs.dt_half;
+    const float h = dt_half;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
     st(acc, s, make_float4(R[0], R[1], R[2], R[3]));
   } else if (mode == MODE_RK4_2 || mode == MODE_RK4_3) {
-    const float h = (mode == MODE_RK4_2) ? cs.dt_half : cs.dt;
+    const float h = (mode == MODE_RK4_2) ? dt_half : dt;
     o = make_float4(w0.x - h * R[0], w0.y - h * R[1], w0.z - h * R[2], w0.w - h * R[3]);
     const float4 a = ld(acc, s);
     st(acc, s, make_float4(a.x + 2.0f * R[0], a.y + 2.0f * R[1], a.z + 2.0f * R[2], a.w + 2.0f * R[3]));
   } else {  // MODE_RK4_4
-    const float h = cs.dt_sixth;
+    const float h = dt_sixth;
     const float4 a = ld(acc, s);
     o = make_float4(w0.x - h * (a.x + R[0]), w0.y - h * (a.y + R[1]), w0.z - h * (a.z + R[2]),
                     w0.w - h * (a.w + R[3]));
@@ -1035,16 +1057,7 @@ __global__ __launch_bounds__(BLOCK) void k_dt(const GroupDev* __restrict__ gdev,
     const int32_t c = r.c0 + t % r.Ck;
     const int32_t cs0 = t / r.Ck;
     for (int32_t cell = r.cell_begin + cs0; cell < r.cell_end; cell += r.P) {
-      const Prim q = prim_of(W[G->st_off + (int64_t)cell * G->C + c], ph.M2, ph.gm1);
-      const float cel = sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM;
-      float tk[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float2 nk = m.nrm[k * m.n + cell];
-        const float un = q.u * nk.x + q.v * nk.y;
-        tk[k] = (cel + fabsf(un)) * m.len[k * m.n + cell];
-      }
-      const float d = m.area[cell] / ((tk[0] + tk[1]) + tk[2]);
+      const float d = cell_dt1(m, ph, cell, W[G->st_off + (int64_t)cell * G->C + c]);
       acc = (d != d) ? d : ((acc != acc) ? acc : fminf(acc, d));
     }
   }

@@ -51,7 +51,7 @@ def get_stage_path() -> str:
 
 class BatchSolver:
     def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
-                 graph=True, autotune=False, launch=None):
+                 graph=True, autotune=False, launch=None, local_dt=False):
         require_gpu()
         self.autotune = bool(autotune)
         self.launch = tuple(launch) if launch else None   # (path, block, xcd_order), applied at prepare
@@ -70,7 +70,9 @@ class BatchSolver:
         so.check_every = max(1, int(check_every))
         so.patience = max(1, int(patience))
         so.max_steps = int(max_steps)
-        so.local_dt = 0
+        # local time stepping (acceleration mode; the reference steps every
+        # cell with one global dt): each cell advances with its own CFL dt
+        so.local_dt = 1 if local_dt else 0
         so.no_graph = 0 if graph else 1
         h = C.c_void_p()
         L.check(L.lib().esr_solver_create(n, arr, C.byref(so), C.byref(h)), "esr_solver_create")

@@ -157,7 +157,16 @@ typedef struct {
   int32_t check_every;       /* stationarity_check_every        */
   int32_t patience;          /* stationarity_patience (>= 1)    */
   int64_t max_steps;         /* extra cap on steps (<= 0: none) */
-  int32_t local_dt;          /* 0: global dt (reference). 1: local CFL dt (acceleration mode) */
+  int32_t local_dt;          /* 0: one global dt per case from its initial state
+                                (main.py:116-122, helper.py:8-18). 1: local time
+                                stepping, an acceleration mode with no reference
+                                counterpart: every cell advances with its own
+                                dt = cfl * area / sum_k (c + |u . n_k|) len_k of
+                                the step's start state (the same dt in every stage;
+                                the global dt is its minimum over cells). A steady
+                                state is the same; the path to it, and therefore
+                                the stationarity residual, is not -- judge such
+                                solves by CD/CL (engineering targets). */
   int32_t no_graph;          /* 1: launch every kernel eagerly instead of replaying
                                 a captured HIP graph of check_every steps */
 } esr_solve_opts;

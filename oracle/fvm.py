@@ -720,16 +720,33 @@ SCHEMES = {"EE": time_step_Euler, "RK2": time_step_RK2, "RK4": time_step_RK4,
            "SRK2": time_step_RK2_SSP, "SSP_RK2": time_step_RK2_SSP}
 
 
-def get_dt(W, om, CFL=0.5, gamma=1.4, M=1.0):
-    """helper.py:8-18"""
+def cell_dt(W, om, gamma=1.4, M=1.0):
+    """area / sum_k (c + |u . n_k|) len_k per cell: helper.py:8-18 before the
+    minimum over cells and the CFL factor."""
     p = getPrimitive(W, gamma=gamma, M=M)
     rho, u, v, P = p[:, 0], p[:, 1], p[:, 2], p[:, 3]
     c = np.sqrt(np.abs(gamma * P / rho)) / M
     un = u[:, None] * om.normals[..., 0] + v[:, None] * om.normals[..., 1]
     lam = c[:, None] + np.abs(un)
     t = lam * om.surface[om.face_connectivity]
-    dt_cell = om.area / ((t[:, 0] + t[:, 1]) + t[:, 2])
-    return np.min(dt_cell) * CFL
+    return om.area / ((t[:, 0] + t[:, 1]) + t[:, 2])
+
+
+def get_dt(W, om, CFL=0.5, gamma=1.4, M=1.0):
+    """helper.py:8-18"""
+    return np.min(cell_dt(W, om, gamma=gamma, M=M)) * CFL
+
+
+def local_dt(W, om, CFL=0.5, gamma=1.4, M=1.0):
+    """Per-cell time steps of local time stepping, shaped (N, 1) to broadcast
+    over the state: cell_dt * CFL from the step's start state. An
+    acceleration mode of this project, not a reference function (the
+    reference steps every cell with get_dt's global minimum); passed as `dt`
+    to the steppers below it gives each cell its own step. Parity unpinned
+    against the reference (it has none); pinned by its minimum, which is
+    get_dt, and by the stepper reducing to the global step when every entry
+    is the global dt."""
+    return (cell_dt(W, om, gamma=gamma, M=M) * CFL)[:, None]
 
 
 # ---------------------------------------------------------------------------

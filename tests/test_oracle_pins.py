@@ -527,3 +527,23 @@ def test_forces_uniform_pressure_vanish_and_upper_panel_closed_form(diamond02):
     cd, cl = F.forces(W, om, 1.0, math.sqrt(2.0), 1.0)
     assert abs(cd) < 1e-12
     assert math.isclose(cl, -1.0 * 1.0, rel_tol=1e-12)
+
+
+def test_local_dt_minimum_is_get_dt_and_constant_steps_reduce_to_the_global_step(diamond02):
+    """Local time stepping (an acceleration mode with no reference
+    counterpart): the per-cell steps' minimum is get_dt's global step bit for
+    bit, the cells' steps differ on a graded mesh, and a stepper fed every
+    cell the global step reproduces the global step bit for bit."""
+    from conftest import base_cfg, perturbed_state
+    cfg = base_cfg()
+    W, inlet = perturbed_state(diamond02, cfg, seed=1)
+    om = F.omesh(diamond02, np.float32)
+    d = F.local_dt(W, om, CFL=0.6)
+    g = F.get_dt(W, om, CFL=0.6)
+    assert d.shape == (diamond02.n_cells, 1) and d.dtype == np.float32
+    assert d.min() == g and d.max() > 2.0 * d.min()
+    kw = dict(gamma=1.4, M=1.0, reconstruction="muscl", flux="hllc", value=inlet, entropy=False)
+    for step in (F.time_step_Euler, F.time_step_RK2, F.time_step_RK2_SSP, F.time_step_RK4):
+        a = step(W, om, g, **kw)
+        b = step(W, om, np.full((diamond02.n_cells, 1), g, dtype=np.float32), **kw)
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), step.__name__
