@@ -115,6 +115,9 @@ struct Engine {
   int scheme = ESR_SCHEME_SRK2;
   int32_t stage_path = ESR_STAGE_CELL;
   int32_t cur_block = BLOCK, cur_xcd = 1;
+  // local time stepping (esr_solve_opts.local_dt): the update kernel's local
+  // variant, on the face path (the cell-centric kernel has none)
+  bool local_dt = false;
   std::vector<uint8_t> group_on;      // empty: every group launched
   std::vector<GroupDev> groups;       // host copies
   std::vector<const esr_mesh*> gmesh; // mesh of each group
@@ -193,7 +196,7 @@ struct Engine {
     block_tables(rec_all, block, xcd);
     block_tables(cell_all, block, xcd);
     reduction_tables();
-    stage_path = path;
+    stage_path = local_dt ? ESR_STAGE_FACE : path;
     cur_block = block;
     cur_xcd = xcd;
   }
@@ -385,7 +388,8 @@ struct Engine {
       if (stage_path == ESR_STAGE_FACE) launch_face(c->ls, cases(), Ws, rec.as<float4>(), fx.as<float4>(), st);
     }
     if (stage_path == ESR_STAGE_FACE)
-      launch_update(cell_all.ls, mode, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(), st);
+      launch_update(cell_all.ls, mode, local_dt, cases(), Ws, W0.as<float4>(), fx.as<float4>(), out, acc.as<float4>(),
+                    st);
     else
       for (auto& c : classes)
         launch_cell(c->ls, mode, cases(), Ws, W0.as<float4>(), rec.as<float4>(), out, acc.as<float4>(), st);
@@ -792,6 +796,8 @@ esr_status esr_solver_create(int32_t n_cases, const esr_case* cases, const esr_s
       // local time stepping: each case's cells advance with their own dt at
       // the case's CFL (the stage kernels read it from the case parameters)
       Engine& e = s->eng;
+      e.local_dt = true;
+      e.stage_path = ESR_STAGE_FACE;
       for (int32_t u = 0; u < n_cases; ++u) {
         if (!(cases[u].cfl > 0.f)) throw Error(ESR_ERR_VALUE, "cfl must be > 0");
         e.cases_h[e.global_of_user[u]].cfl_local = cases[u].cfl;
@@ -1189,9 +1195,11 @@ esr_status esr_solver_autotune(esr_solver* s, int32_t reps, int32_t* path, int32
       // every configuration timed in two sweeps (the best of the two counts),
       // so that clock ramps and neighbours' noise do not pick a slower one
       std::vector<TuneChoice> cand;
-      for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE})
+      for (int32_t p : {ESR_STAGE_CELL, ESR_STAGE_FACE}) {
+        if (e.local_dt && p == ESR_STAGE_CELL) continue;  // local time stepping runs the face path
         for (int32_t b : {64, 128, 256})
           for (int32_t x : {1, 0}) cand.push_back(TuneChoice{p, b, x, INFINITY});
+      }
       for (int sweep = 0; sweep < 2; ++sweep) {
         for (TuneChoice& c : cand) {
           e.configure(c.path, c.block, c.xcd, st);
@@ -1231,6 +1239,8 @@ esr_status esr_solver_set_launch(esr_solver* s, int32_t path, int32_t block, int
     if (!s) throw Error(ESR_ERR_VALUE, "null solver");
     if (path != ESR_STAGE_CELL && path != ESR_STAGE_FACE)
       throw Error(ESR_ERR_VALUE, "stage path must be ESR_STAGE_CELL or ESR_STAGE_FACE");
+    if (path == ESR_STAGE_CELL && s->eng.local_dt)
+      throw Error(ESR_ERR_VALUE, "local time stepping runs the face path (ESR_STAGE_FACE)");
     if (block != 64 && block != 128 && block != 256) throw Error(ESR_ERR_VALUE, "block must be 64, 128 or 256");
     if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
     s->eng.configure(path, block, xcd_order ? 1 : 0, S(stream));
@@ -1263,7 +1273,8 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
                                              e.fx.as<float4>(), st);
     };
     auto update = [&] {
-      launch_update(e.cell_all.ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.fx.as<float4>(),
+      launch_update(e.cell_all.ls, MODE_S1, e.local_dt, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(),
+                    e.fx.as<float4>(),
                     e.W1.as<float4>(), e.acc.as<float4>(), st);
     };
     auto cell = [&] {

@@ -791,6 +791,9 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
 // sponge source, and the stage update selected by `mode` (a StageMode)
 // (s: the cell's element offset in the group slabs W0, out and acc, which
 // are the group's base pointers)
+// LOCAL: local time stepping (each cell's own dt, CaseDev.cfl_local); a
+// compile-time choice, so the global-step kernels carry none of its code.
+template <bool LOCAL>
 __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell, uint32_t s,
                                                const float4 Wi, const float4 Fk[3], const float4* W0, float4* out,
                                                float4* acc) {
@@ -816,7 +819,7 @@ __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, cons
   // time step: the case's global dt, or with local time stepping the cell's
   // own dt from the step's start state w0 (the same dt in every stage)
   float dt = cs.dt, dt_half = cs.dt_half, dt_sixth = cs.dt_sixth;
-  if (cs.cfl_local > 0.0f) {
+  if (LOCAL) {
     dt = cell_dt1(G->m, G->ph, cell, w0) * cs.cfl_local;
     dt_half = dt / 2.0f;
     dt_sixth = dt / 6.0f;
@@ -848,7 +851,7 @@ __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, cons
   st(out, s, o);
 }
 
-template <int MODE>
+template <int MODE, bool LOCAL>
 __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ groups,
                                                   const BlockRef* __restrict__ blk_group,
                                                   const CaseDev* __restrict__ cases,
@@ -869,7 +872,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
     Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
   const int64_t so = G->st_off;
-  stage_epilogue(MODE, G, cs, cell, it.e, ld(Ws + so, it.e), Fk, W0 + so, out + so, acc ? acc + so : acc);
+  stage_epilogue<LOCAL>(MODE, G, cs, cell, it.e, ld(Ws + so, it.e), Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -932,7 +935,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
     Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
   }
   const int64_t so = G->st_off;
-  stage_epilogue(mode, G, cs, cell, it.e, Wi, Fk, W0 + so, out + so, acc ? acc + so : acc);
+  stage_epilogue<false>(mode, G, cs, cell, it.e, Wi, Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1430,13 +1433,13 @@ void launch_cell(const LaunchSet& ls, int mode, const CaseDev* cases, const floa
   check_launch("k_cell");
 }
 
-void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                   const float4* fx, float4* out, float4* acc, hipStream_t st) {
-  if (ls.nblocks <= 0) return;
-#define ESR_UPD_CASE(MD)                                                                                   \
-  case MD:                                                                                                 \
-    hipLaunchKernelGGL((k_update<MD>), dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, \
-                       Ws, W0, fx, out, acc, ls.xcd);                                                              \
+template <bool LOCAL>
+static void launch_update_l(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
+                            const float4* fx, float4* out, float4* acc, hipStream_t st) {
+#define ESR_UPD_CASE(MD)                                                                                     \
+  case MD:                                                                                                   \
+    hipLaunchKernelGGL((k_update<MD, LOCAL>), dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, \
+                       cases, Ws, W0, fx, out, acc, ls.xcd);                                                 \
     break;
   switch (mode) {
     ESR_UPD_CASE(MODE_RESID)
@@ -1452,6 +1455,15 @@ void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const fl
       throw Error(ESR_ERR_VALUE, "unknown stage mode");
   }
 #undef ESR_UPD_CASE
+}
+
+void launch_update(const LaunchSet& ls, int mode, bool local_dt, const CaseDev* cases, const float4* Ws,
+                   const float4* W0, const float4* fx, float4* out, float4* acc, hipStream_t st) {
+  if (ls.nblocks <= 0) return;
+  if (local_dt)
+    launch_update_l<true>(ls, mode, cases, Ws, W0, fx, out, acc, st);
+  else
+    launch_update_l<false>(ls, mode, cases, Ws, W0, fx, out, acc, st);
   check_launch("k_update");
 }
 

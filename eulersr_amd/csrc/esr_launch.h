@@ -34,8 +34,10 @@ void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws,
                     hipStream_t st);
 void launch_face(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec, float4* fx,
                  hipStream_t st);
-void launch_update(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
-                   const float4* fx, float4* out, float4* acc, hipStream_t st);
+// local_dt: every cell with its own dt (CaseDev.cfl_local; the cell-centric
+// kernel has no such variant, so local time stepping runs the face path)
+void launch_update(const LaunchSet& ls, int mode, bool local_dt, const CaseDev* cases, const float4* Ws,
+                   const float4* W0, const float4* fx, float4* out, float4* acc, hipStream_t st);
 // cell-centric stage (face fluxes of each cell + residual + update in one
 // launch); the alternative to launch_face + launch_update
 void launch_cell(const LaunchSet& ls, int mode, const CaseDev* cases, const float4* Ws, const float4* W0,
