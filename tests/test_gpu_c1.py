@@ -1,0 +1,61 @@
+"""CD/CL of the CPU-runnable configuration after a long solve, against the
+oracle: diamond at h = 0.1, M = 2.5, alpha = 5 deg, HLLC + MUSCL, SSP-RK2,
+CFL 0.5, run until the stationarity residual drops below 1e-6 (checked every
+5 steps, patience 1) or 40,000 steps. (Measured: the residual settles on the
+K = 0 limiter's plateau near 7e-6, so both geometries run the full 40,000
+steps.) Two geometries: the configuration's own (t/c = 0.1, i.e. half-angle
+5.71 deg, in the box [-5, 6] x [-5, 5], with p_inf = 1/gamma) and the
+reference's default (half-angle 5 deg in [0, 4]^2, p_inf = 1). The GPU solve
+stops where its loop says; the C restatement of the oracle (oracle/cfvm.c)
+then advances the same freestream state the same number of steps with the
+same dt, and CD and CL must agree within 1e-4 (measured: 1e-6)."""
+import numpy as np
+import pytest
+
+from conftest import base_cfg
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = {
+    "configuration": dict(mesh=dict(Lx=11.0, Ly=10.0, x0=-5.0, y0=-5.0, cx=0.5, cy=0.0, height=0.1),
+                          p_inf=1.0 / 1.4),
+    "reference_default": dict(mesh=dict(), p_inf=1.0),
+}
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_c1_long_solve_forces_match_oracle(variant):
+    from eulersr_amd import main as M
+    from eulersr_amd.meshing import build_diamond
+    from eulersr_amd.solve import BatchSolver
+    from oracle import cfvm as CF
+    from oracle import fvm as F
+    v = VARIANTS[variant]
+    mesh = build_diamond(h=0.1, **v["mesh"])
+    cfg = base_cfg(Mach=2.5, aoa=5.0, p_inf=v["p_inf"], CFL=0.5)
+    W0, inlet = M.initialize(mesh, cfg)
+    sp = M.case_spec(mesh, inlet, cfg)
+    sp.tf = 1.0e6
+    print(f"case: C1 {variant}: {mesh.n_cells} cells", flush=True)
+    bs = BatchSolver([sp], scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=40000)
+    try:
+        bs.set_state(0, W0)
+        bs.prepare()
+        bs.run()
+        bs.finalize()
+        rep = bs.report(0)
+        Wg = bs.get_state(0).cpu().numpy()
+    finally:
+        bs.close()
+    steps = rep["steps_run"]
+    print(f"GPU: converged={rep['converged']} stopping_step={rep['stopping_step']} steps_run={steps} "
+          f"rel={rep['last_rel']:.3e} CD={rep['cd']:.6f} CL={rep['cl']:.6f}", flush=True)
+    assert steps > 1000
+    om = F.omesh(mesh, np.float32)
+    kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=inlet, entropy=False)
+    Wr = CF.srk2_steps(W0, CF.CMesh(om), np.float32(rep["dt"]), steps, **kw)
+    U = 2.5 * np.sqrt(1.4 * v["p_inf"])
+    cd, cl = F.forces(Wr, om, 1.0, U, mesh.metadata["obstacle_length"])
+    dmax = float(np.max(np.abs(F.getPrimitive(Wg.astype(np.float64)) - F.getPrimitive(Wr.astype(np.float64)))))
+    print(f"oracle after {steps} steps: CD={cd:.6f} CL={cl:.6f}; max |dprim| {dmax:.2e}", flush=True)
+    assert abs(rep["cd"] - cd) <= 1e-4 and abs(rep["cl"] - cl) <= 1e-4
