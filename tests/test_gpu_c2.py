@@ -74,3 +74,42 @@ def test_c2_cases_100_steps_within_1e5(launch):
               f"{np.array2string(rel.max(axis=0), precision=2)}", flush=True)
         assert np.all(np.isfinite(pg))
         assert rel.max() <= 1e-5
+
+
+def test_largest_and_smallest_mesh_in_one_launch_within_1e5():
+    """The extremes of the ragged configuration's mesh sizes in one launch:
+    the diamond at h = 0.0125 (89k cells) and at h = 0.1 (1.6k cells),
+    100 SSP-RK2 steps from freestream, every primitive of every cell within
+    1e-5 of the C oracle given the same dt."""
+    from eulersr_amd import main as M
+    from eulersr_amd.solve import BatchSolver
+    from oracle import cfvm as CF
+    from oracle import fvm as F
+    specs, inits = [], []
+    for h, mach in ((0.0125, 2.0), (0.1, 1.5)):
+        mesh = mesh_of("diamond", h)
+        cfg = base_cfg(Mach=mach, aoa=AOA, case="diamond", flux="HLLC", reconstruction="MUSCL", CFL=0.6)
+        W, inlet = M.initialize(mesh, cfg)
+        specs.append(M.case_spec(mesh, inlet, cfg))
+        inits.append(W)
+    print(f"cells {[sp.mesh.n_cells for sp in specs]}", flush=True)
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=1, max_steps=STEPS)
+    try:
+        for i, W in enumerate(inits):
+            bs.set_state(i, W)
+        bs.prepare()
+        bs.run()
+        reps = bs.reports()
+        states = [bs.get_state(i).cpu().numpy() for i in range(len(specs))]
+    finally:
+        bs.close()
+    for i, (sp, W0) in enumerate(zip(specs, inits)):
+        kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=sp.kw["value"], entropy=False)
+        Wr = CF.srk2_steps(W0, CF.CMesh(F.omesh(sp.mesh, np.float32)), np.float32(reps[i]["dt"]), STEPS, **kw)
+        pg = F.getPrimitive(states[i].astype(np.float64))
+        pr = F.getPrimitive(Wr.astype(np.float64))
+        U = np.hypot(sp.kw["value"][1], sp.kw["value"][2]) / sp.kw["value"][0]
+        rel = np.abs(pg - pr) / np.maximum(np.abs(pr), np.array([1.0, U, U, 1.0]))
+        print(f"{sp.mesh.n_cells} cells: max rel per primitive {np.array2string(rel.max(axis=0), precision=2)}",
+              flush=True)
+        assert reps[i]["steps_run"] == STEPS and rel.max() <= 1e-5
