@@ -285,6 +285,22 @@ def rate(run):
     return sum(x["converged"] for x in run["reports"]) / run["total_s"] * 3600.0
 
 
+def occupancy(trace, slots):
+    """Slot use of a campaign from its per-chunk trace (seconds, step, cases in
+    flight, cases waiting): time-weighted mean cases in flight, and the time
+    after the queue emptied (the tail the slowest cases set)."""
+    if len(trace) < 2:
+        return None
+    t = np.array([x[0] for x in trace])
+    act = np.array([x[2] for x in trace], dtype=np.float64)
+    wait = np.array([x[3] for x in trace])
+    dt = np.diff(t)
+    empty = np.nonzero(wait == 0)[0]
+    t_empty = float(t[empty[0]]) if empty.size else float(t[-1])
+    return dict(mean_cases_in_flight=float(np.sum(act[1:] * dt) / max(np.sum(dt), 1e-12)), slots=slots,
+                queue_empty_at_s=t_empty, tail_s=float(t[-1] - t_empty), hr_s=float(t[-1]))
+
+
 def small_config(meshes, starter, args):
     """The reference's single-case GPU configuration (6 cases) measured the
     same way: batched warm start, batched cold start, plain single-case."""
@@ -403,8 +419,10 @@ def main():
         log(f"warmup campaign {w + 1}/{args.warmup}")
         starter.campaign(cases, mode="mlp", slots=args.slots, max_steps=args.max_steps)
     times, runs = [], []
+    trace = []
     for k in range(args.steps):
-        r = starter.campaign(cases, mode="mlp", slots=args.slots, max_steps=args.max_steps)
+        trace = []
+        r = starter.campaign(cases, mode="mlp", slots=args.slots, max_steps=args.max_steps, trace=trace)
         times.append(r["total_s"])
         runs.append(r)
         conv = sum(x["converged"] for x in r["reports"])
@@ -413,6 +431,10 @@ def main():
             f"inference {r['infer_s'] * 1e3:.1f}ms, HR {r['hr_s']:.2f}s, {args.slots} slots) "
             f"converged={conv}/{len(cases)} steps median {np.median(st) if st else 0:.0f} "
             f"mean {np.mean(st) if st else 0:.0f} max {max(st) if st else 0} fallback={sum(r['fallback'])}")
+    occ = occupancy(trace, args.slots)
+    if occ:
+        log(f"  HR slots in use: {occ['mean_cases_in_flight']:.1f} of {args.slots} on average; queue empty at "
+            f"{occ['queue_empty_at_s']:.1f}s, tail {occ['tail_s']:.1f}s")
     T = sum(times)
     n_conv = sum(sum(x["converged"] for x in r["reports"]) for r in runs)
     value = n_conv / T * 3600.0
@@ -485,6 +507,7 @@ def main():
         "converged": int(sum(x["converged"] for x in warm["reports"])),
         "fallback": int(sum(warm["fallback"])),
         "pipeline_split_s": {"lr": warm["lr_s"], "inference": warm["infer_s"], "hr": warm["hr_s"]},
+        "campaign_occupancy": occupancy(trace, args.slots),
         "steps_warm": {"median": float(np.median([x["stopping_step"] for x in warm["reports"]])),
                        "mean": float(np.mean([x["stopping_step"] for x in warm["reports"]])),
                        "max": int(max(x["stopping_step"] for x in warm["reports"]))},

@@ -467,18 +467,20 @@ class WarmStarter:
             t_inf = time.perf_counter() - t0
         return inits, fallback, t_lr, t_inf
 
-    def campaign(self, cases, mode="mlp", slots=128, max_steps=0, lr_max_steps=0):
+    def campaign(self, cases, mode="mlp", slots=128, max_steps=0, lr_max_steps=0, trace=None):
         """The warm-start protocol over a campaign of cases with continuous
         batching (`campaign`): LR solves and predictions for every case, then
         the HR solves, `slots` at a time. A warm-started case that diverges
-        is solved again from freestream. Returns a summary like `run`."""
+        is solved again from freestream. Returns a summary like `run`;
+        `trace` receives the HR campaign's (seconds, step, cases in flight,
+        cases waiting) after every chunk."""
         import torch
         torch.cuda.synchronize()
         t_all = time.perf_counter()
         inits, fallback, t_lr, t_inf = self.initial_states(cases, mode, lr_max_steps)
         fb = None if mode == "cold" else [c.hr_init for c in cases]
         reps, t, restarted = campaign([c.hr_spec for c in cases], inits, slots=slots, max_steps=max_steps,
-                                      fallback_inits=fb)
+                                      fallback_inits=fb, trace=trace)
         for i in restarted:
             fallback[i] = True
         torch.cuda.synchronize()
