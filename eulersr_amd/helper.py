@@ -74,6 +74,19 @@ def get_force_coefficients(W, mesh, rho_inf, U_inf, L_ref):
     return _forces(W, mesh, rho_inf, U_inf, L_ref)
 
 
+def relative_change(W_next, W, mesh):
+    """||W_next - W||_F / (||W||_F + 1e-16) on the GPU, in float64: the
+    one-step probe residual of main.py:228-230 and
+    eval/_fvm_solve_worker.py:133-134."""
+    dm = device_mesh(mesh)
+    a, _ = as_device_state(W_next)
+    b, _ = as_device_state(W)
+    d, r = C.c_double(), C.c_double()
+    L.check(L.lib().esr_norms(dm.handle, C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), C.byref(d),
+                              C.byref(r), stream_ptr()), "esr_norms")
+    return d.value / (r.value + 1e-16)
+
+
 def get_total_entropy(W, mesh, gamma=1.4):
     """helper.py:312-315 (host diagnostic)."""
     W = np.asarray(W, dtype=np.float64)
