@@ -30,7 +30,8 @@ sub-groups handed to the kinds with waiting cases (pipeline.campaign).
 ``value`` = converged cases per hour over the K timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
 the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
 the warm start against those plain solves, the same campaign from cold
-starts, the stage kernels' HBM rates at the campaign's 64-case launch, and the
+starts, the same campaign warm-started by IDW alone (no network), the stage
+kernels' HBM rates at the campaign's 64-case launch, and the
 reference's single-case configuration (6 cases: diamond + NACA0012, M in
 {0.9, 1.1, 2.0}, alpha 2 deg) measured the same way. The roofline object is
 for the slowest stage kernel of the autotuned launch of the campaign's
@@ -456,12 +457,18 @@ def main():
         drift = [abs(a - p["cd"]) for a, p in zip(later, single)]
         log(f"|dCD| warm vs plain: median {np.median(dcd):.1e}, within 1e-4: {sum(d <= 1e-4 for d in dcd)}/"
             f"{len(dcd)}; plain CD change to 2x its steps: median {np.median(drift):.1e}")
-    cold = None
+    cold = idw = None
     if not args.profile:
         cold = starter.campaign(cases, mode="cold", slots=args.slots, max_steps=args.max_steps)
         cst = [x["stopping_step"] for x in cold["reports"] if x["converged"]]
         log(f"cold-start campaign: {cold['total_s']:.2f}s -> {rate(cold):.0f} cases/hour, steps median "
             f"{np.median(cst) if cst else 0:.0f} mean {np.mean(cst) if cst else 0:.0f}")
+        # the same warm-start pipeline without the network (IDW prediction only)
+        idw = starter.campaign(cases, mode="idw", slots=args.slots, max_steps=args.max_steps)
+        ist = [x["stopping_step"] for x in idw["reports"] if x["converged"]]
+        log(f"IDW warm-start campaign: {idw['total_s']:.2f}s -> {rate(idw):.0f} cases/hour, steps median "
+            f"{np.median(ist) if ist else 0:.0f} mean {np.mean(ist) if ist else 0:.0f}, "
+            f"fallback={sum(idw['fallback'])}")
 
     # the stage kernels at the campaign's launch: its first `slots` cases,
     # from their warm-start states
@@ -502,6 +509,11 @@ def main():
         "plain_single_case_cases_per_hour": single_rate,
         "plain_single_case_sample": len(sample),
         "cold_campaign_cases_per_hour": rate(cold) if cold else None,
+        "idw_campaign_cases_per_hour": rate(idw) if idw else None,
+        "steps_idw": None if not idw else {
+            "median": float(np.median([x["stopping_step"] for x in idw["reports"]])),
+            "mean": float(np.mean([x["stopping_step"] for x in idw["reports"]])),
+            "fallback": int(sum(idw["fallback"]))},
         "slots": args.slots,
         "cell_updates_per_s": cell_updates / warm["hr_s"],
         "converged": int(sum(x["converged"] for x in warm["reports"])),
