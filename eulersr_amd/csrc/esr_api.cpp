@@ -1261,6 +1261,21 @@ esr_status esr_set_stage_path(int32_t path) {
 
 int32_t esr_get_stage_path(void) { return resolved_stage_path(); }
 
+esr_status esr_check_device_math(uint64_t* counts, void* stream) {
+  return guard([&] {
+    if (!counts) throw Error(ESR_ERR_VALUE, "counts is null");
+    hipStream_t st = S(stream);
+    DevBuf d;
+    d.alloc(4 * sizeof(unsigned long long));
+    hip_check(hipMemsetAsync(d.p, 0, 4 * sizeof(unsigned long long), st), "memset");
+    launch_math_check(d.as<unsigned long long>(), st);
+    unsigned long long h[4];
+    hip_check(hipMemcpyAsync(h, d.p, sizeof(h), hipMemcpyDeviceToHost, st), "copy");
+    hip_check(hipStreamSynchronize(st), "sync");
+    for (int k = 0; k < 4; ++k) counts[k] = (uint64_t)h[k];
+  });
+}
+
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream) {
   return guard([&] {
     if (!s || reps <= 0 || !ms) throw Error(ESR_ERR_VALUE, "bad arguments");

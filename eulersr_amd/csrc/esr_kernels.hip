@@ -41,6 +41,40 @@ namespace esr {
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
 
+// Correctly rounded square root, the value of sqrtf: the hardware v_sqrt_f32
+// result s is within one ulp, and the sign of the residual x - s' s of each
+// neighbour s' = s -/+ 1 ulp (one fma each) tells whether the neighbour is the
+// rounded root. sqrtf's generic lowering adds a rescaling of inputs below
+// 2^-96 (to keep denormals exact) and zero/infinity fix-ups around the same
+// test; inputs here (c^2 = gamma P / rho, Roe densities clamped to >= 1e-6,
+// rho / P) are normal, and +0, +inf and NaN come out right without them.
+__device__ __forceinline__ float fsqrt(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+  const float su = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rd = __builtin_fmaf(-sd, s, x);
+  const float ru = __builtin_fmaf(-su, s, x);
+  const float r = (rd <= 0.0f) ? sd : s;
+  return (ru > 0.0f) ? su : r;
+}
+
+// Natural logarithm, the value of logf for inputs of at least 2^-96: log2 x
+// from the hardware v_log_f32, times ln 2 split as LN2_HI + LN2_LO (LN2_HI =
+// ln 2 to 24 bits rounded toward zero, LN2_LO the float nearest the rest), the
+// product y ln 2 formed with its rounding error (an fma) and rounded once.
+// Zero and infinities pass through as log2 gives them (-inf, +inf), NaN stays
+// NaN; logf's rescaling of inputs below 2^-96 is left out (the inputs here are
+// pressures, densities and their ratios).
+__device__ __forceinline__ float flog(float x) {
+  constexpr float LN2_HI = 0x1.62e42ep-1f, LN2_LO = 0x1.efa39ep-25f;
+  const float y = __builtin_amdgcn_logf(x);
+  const float r = y * LN2_HI;
+  float e = __builtin_fmaf(y, LN2_HI, -r);
+  e = __builtin_fmaf(y, LN2_LO, e);
+  const float l = __builtin_fmaf(y, LN2_HI, e);
+  return (fabsf(y) < __builtin_inff()) ? l : y;
+}
+
 // ---------------------------------------------------------------------------
 // state conversions (helper.py:25-46)
 // ---------------------------------------------------------------------------
@@ -201,7 +235,7 @@ __device__ __forceinline__ uint32_t kn(int k, int32_t n, int32_t cell) {
 // local time stepping uses it per cell.
 __device__ __forceinline__ float cell_dt1(const MeshDev& m, const Phys& ph, int32_t cell, const float4 W) {
   const Prim q = prim_of(W, ph.M2, ph.gm1);
-  const float cel = sqrtf(fabsf(ph.g * q.P / q.r)) * ph.invM;
+  const float cel = fsqrt(fabsf(ph.g * q.P / q.r)) * ph.invM;
   float tk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -330,7 +364,7 @@ __device__ __forceinline__ void phys_flux(const float4 W, const Prim& q, float u
 }
 
 __device__ __forceinline__ float sound(const Prim& q, const Phys& ph) {
-  return __builtin_sqrtf(fabsf(ph.g * q.P * frcp(q.r))) * ph.invM;
+  return fsqrt(fabsf(ph.g * q.P * frcp(q.r))) * ph.invM;
 }
 
 // HLLC flux F_HLLC (euler_solver.py:273-372); FL, FR receive the physical
@@ -403,7 +437,7 @@ __device__ __forceinline__ float4 entropy_vars(const float4 W, const Phys& ph) {
   const float r = W.x, ir = frcp(r), u = W.y * ir, v = W.z * ir;
   const float P = (W.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
   const float iP = frcp(P);
-  const float s = logf(P) - ph.g * logf(r);  // ln(P / rho^gamma)
+  const float s = flog(P) - ph.g * flog(r);  // ln(P / rho^gamma)
   float4 V;
   V.x = (ph.g - s) * ph.inv_gm1 - 0.5f * r * (u * u + v * v) * iP;
   V.y = W.y * iP;
@@ -444,7 +478,7 @@ __device__ __forceinline__ float log_mean(float aL, float aR) {
   const float f = fdiv(xi - 1.0f, xi + 1.0f);
   const float u = f * f;
   const float F = (u < 1e-3f) ? 1.0f + u * (1.0f / 3.0f) + (u * u) * 0.2f + (u * (u * u)) * (1.0f / 7.0f)
-                              : fdiv(logf(xi), 2.0f * f);
+                              : fdiv(flog(xi), 2.0f * f);
   return fdiv(aL + aR, 2.0f * F);
 }
 
@@ -455,7 +489,7 @@ __device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR
   {
     const float r = WL.x, ir = frcp(r), u = WL.y * ir, v = WL.z * ir;
     const float P = (WL.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
-    const float s = __builtin_sqrtf(fdiv(r, P));
+    const float s = fsqrt(fdiv(r, P));
     ZL[0] = s;
     ZL[1] = s * u;
     ZL[2] = s * v;
@@ -464,7 +498,7 @@ __device__ __forceinline__ void ismail_roe_flux(const float4 WL, const float4 WR
   {
     const float r = WR.x, ir = frcp(r), u = WR.y * ir, v = WR.z * ir;
     const float P = (WR.w - 0.5f * r * (u * u + v * v)) * ph.gm1;
-    const float s = __builtin_sqrtf(fdiv(r, P));
+    const float s = fsqrt(fdiv(r, P));
     ZR[0] = s;
     ZR[1] = s * u;
     ZR[2] = s * v;
@@ -516,7 +550,7 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const Prim& L, const f
   }
   PL = (PL <= 0.0f) ? 1e-6f : PL;
   PR = (PR <= 0.0f) ? 1e-6f : PR;
-  const float sL = __builtin_sqrtf(rL), sR = __builtin_sqrtf(rR);
+  const float sL = fsqrt(rL), sR = fsqrt(rR);
   const float isum = frcp(sL + sR);
   const float r = sL * sR;
   const float u = (sL * uL + sR * uR) * isum;
@@ -524,7 +558,7 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const Prim& L, const f
   const float HL = (WL.w + PL) * irL;
   const float HR = (WR.w + PR) * irR;
   const float H = (sL * HL + sR * HR) * isum;
-  const float a = __builtin_sqrtf(fabsf(ph.gm1 * (H - 0.5f * (u * u + v * v))));
+  const float a = fsqrt(fabsf(ph.gm1 * (H - 0.5f * (u * u + v * v))));
   // eigenvectors R (columns), euler_solver.py:156-175
   float Rm[4][4];
   Rm[0][0] = 1.0f;
@@ -554,7 +588,7 @@ __device__ __forceinline__ void diss_roe(const float4 WL, const Prim& L, const f
   {
     const float P0 = L.P * ph.invM2, P1 = R.P * ph.invM2;
     const float iP0 = frcp(P0), iP1 = frcp(P1);
-    const float ds = logf(P1 * iP0) - ph.g * logf(R.r * i0);
+    const float ds = flog(P1 * iP0) - ph.g * flog(R.r * i0);
     const float k0 = L.r * (L.u * L.u + L.v * L.v), k1 = R.r * (R.u * R.u + R.v * R.v);  // rho |u|^2
     dv[0] = -ds * ph.inv_gm1 - 0.5f * (k1 * iP1 - k0 * iP0);
     dv[1] = WR.y * iP1 - WL.y * iP0;
@@ -1240,6 +1274,29 @@ __global__ void k_scatter_out(const GroupDev* __restrict__ gdev, int32_t c, cons
   user[perm[i]] = slab[G->st_off + (int64_t)i * G->C + c];
 }
 
+// Exhaustive check of fsqrt and flog against sqrtf and logf over the 2^31
+// non-negative float bit patterns (work-item t takes t, t + T, t + 2T, ...
+// for T work-items in all): counts[0] square-root mismatches at inputs of at
+// least 2^-96, counts[1] below; counts[2], counts[3] the same for the
+// logarithm. NaN equals NaN.
+__global__ __launch_bounds__(BLOCK) void k_math_check(unsigned long long* counts, uint32_t per_item) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t T = gridDim.x * blockDim.x;
+  unsigned long long c[4] = {0, 0, 0, 0};
+  for (uint32_t i = 0; i < per_item; ++i) {
+    const uint32_t b = t + i * T;
+    if (b > 0x7FFFFFFFu) break;
+    const float x = __uint_as_float(b);
+    const int tiny = (b < 0x0F800000u) ? 1 : 0;  // below 2^-96
+    const float s0 = fsqrt(x), s1 = sqrtf(x), l0 = flog(x), l1 = logf(x);
+    if (__float_as_uint(s0) != __float_as_uint(s1) && !(s0 != s0 && s1 != s1)) c[tiny] += 1;
+    if (__float_as_uint(l0) != __float_as_uint(l1) && !(l0 != l0 && l1 != l1)) c[2 + tiny] += 1;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (c[k]) atomicAdd(&counts[k], c[k]);
+}
+
 __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -1556,6 +1613,14 @@ void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src,
   hipLaunchKernelGGL(k_commit, dim3(ls.nblocks), dim3(ls.block), 0, st, ls.groups, ls.blk_group, cases, src, dst,
                      ls.xcd);
   check_launch("k_commit");
+}
+
+void launch_math_check(unsigned long long* counts, hipStream_t st) {
+  // 2^15 blocks x 256 work-items x 256 patterns each = 2^31 patterns
+  constexpr uint32_t blocks = 1u << 15, per_item = 1u << 8;
+  static_assert((uint64_t)blocks * BLOCK * per_item == (1ull << 31), "covers every non-negative pattern");
+  hipLaunchKernelGGL(k_math_check, dim3(blocks), dim3(BLOCK), 0, st, counts, per_item);
+  check_launch("k_math_check");
 }
 
 void launch_copy(const float4* src, float4* dst, int64_t n, hipStream_t st) {

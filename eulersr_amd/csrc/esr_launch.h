@@ -100,6 +100,9 @@ void launch_probe_final(int32_t n_cases, CaseDev* cases, const double* part_d, c
 // commit out -> W0 for the active work-items of a launch set (explicit Euler)
 void launch_commit(const LaunchSet& ls, const CaseDev* cases, const float4* src, float4* dst, hipStream_t st);
 void launch_copy(const float4* src, float4* dst, int64_t n, hipStream_t st);
+// exhaustive check of the flux kernels' square root and logarithm against
+// sqrtf / logf; counts: 4 zeroed device uint64 (see esr_check_device_math)
+void launch_math_check(unsigned long long* counts, hipStream_t st);
 void launch_set_active(int32_t n_cases, CaseDev* cases, const CaseLoop* loops, int32_t all, hipStream_t st);
 void launch_zero_lf(int32_t n_cases, CaseDev* cases, hipStream_t st);
 void launch_count_active(int32_t n_cases, const CaseDev* cases, int32_t* count, hipStream_t st);

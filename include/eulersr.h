@@ -284,6 +284,16 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
  * the stage buffer only). ms holds 4 entries. */
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
 
+/* Exhaustive self-check of the device arithmetic: the square root and the
+ * natural logarithm that the flux kernels evaluate (sound speeds, Roe
+ * averages, entropy jumps) against sqrtf and logf, bit for bit, over all 2^31
+ * non-negative float bit patterns. counts[0] = square-root mismatches at
+ * inputs of at least 2^-96, counts[1] = below 2^-96 (tiny and denormal
+ * inputs, which those quantities never take); counts[2], counts[3] the same
+ * for the logarithm. Synchronous on `stream`. No counterpart in the reference
+ * (a check of this implementation's arithmetic). */
+esr_status esr_check_device_math(uint64_t* counts, void* stream);
+
 /* Stage path of the engines built after the call (solvers created, and the
  * per-mesh engine of esr_residual / esr_step at its next call):
  * ESR_STAGE_CELL -- per stage a reconstruction launch and one cell-centric
