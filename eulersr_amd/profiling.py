@@ -2,6 +2,7 @@
 
     python -m eulersr_amd.profiling trace <dir>/<name>_kernel_trace.csv
     python -m eulersr_amd.profiling pmc <fetch_counter_collection.csv> <write_counter_collection.csv>
+    python -m eulersr_amd.profiling valu <counter_collection.csv>
 
 ``trace``: per kernel, the average duration over all dispatches and over the
 last ``--tail`` dispatches (bench.py ends with back-to-back launches of each
@@ -9,6 +10,10 @@ stage kernel, the same launches it times with HIP events).
 ``pmc``: per kernel, FETCH_SIZE and WRITE_SIZE (KiB per dispatch), and the
 same counters for a device copy of known size (bench.py --profile ends with
 two 1 GiB copies), to give a calibration factor.
+``valu``: per stage kernel, from one pass of SQ_WAVES, SQ_INSTS_VALU,
+SQ_INSTS_VMEM_RD, SQ_WAVE_CYCLES and SQ_BUSY_CU_CYCLES, the means over the
+last ``--tail`` dispatches of waves, VALU and vector-memory read
+instructions per wave, and resident waves per busy CU.
 """
 from __future__ import annotations
 
@@ -87,6 +92,34 @@ def pmc_summary(fetch_csv, write_csv, copy_bytes=None, calib_kernel="__amd_roccl
     return "\n".join(out)
 
 
+STAGE_KERNELS = ("k_recon", "k_cell", "k_face", "k_update")
+
+
+def valu_summary(path, tail=50):
+    """Executed VALU work of the stage kernels (see the module docstring)."""
+    rows = list(csv.DictReader(open(path)))
+    by = defaultdict(dict)
+    name = {}
+    for r in rows:
+        d = int(r["Dispatch_Id"])
+        name[d] = r["Kernel_Name"]
+        by[d][r["Counter_Name"]] = float(r["Counter_Value"])
+    per = defaultdict(list)
+    for d in sorted(by):
+        per[short(name[d])].append(by[d])
+    out = [f"{'kernel':34s} {'waves':>8s} {'VALU/wave':>10s} {'VMEM_RD/wave':>13s} {'waves/busy CU':>14s}"]
+    for k in sorted(per):
+        if not k.split("<")[0] in STAGE_KERNELS:
+            continue
+        t = per[k][-tail:]
+        waves = statistics.mean(c["SQ_WAVES"] for c in t)
+        valu = statistics.mean(c["SQ_INSTS_VALU"] / c["SQ_WAVES"] for c in t)
+        vmem = statistics.mean(c["SQ_INSTS_VMEM_RD"] / c["SQ_WAVES"] for c in t)
+        occ = statistics.mean(c["SQ_WAVE_CYCLES"] / c["SQ_BUSY_CU_CYCLES"] for c in t)
+        out.append(f"{k:34s} {waves:8.0f} {valu:10.1f} {vmem:13.1f} {occ:14.2f}")
+    return "\n".join(out)
+
+
 def read_traffic_label(path):
     """First header line of a traffic file (the run's launch configuration)."""
     with open(path) as f:
@@ -119,9 +152,14 @@ def main():
     p.add_argument("--tail", type=int, default=50)
     p.add_argument("--traffic-out", default=None)
     p.add_argument("--label", default="")
+    v = sub.add_parser("valu")
+    v.add_argument("csv")
+    v.add_argument("--tail", type=int, default=50)
     a = ap.parse_args()
     if a.cmd == "trace":
         print(trace_summary(a.csv, a.tail))
+    elif a.cmd == "valu":
+        print(valu_summary(a.csv, a.tail))
     else:
         print(pmc_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.calib_kernel, a.tail, a.traffic_out, a.label))
 
