@@ -41,6 +41,16 @@ algorithmic bytes per launch come from the launch's work-items
 (``recon_bytes``, ``face_bytes``, ``update_bytes``, ``cell_bytes``).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+``--learned-config`` measures the learned warm start on its own training
+configuration instead (one JSON line, not the headline): 4,000
+solver-generated LR (h = 0.1) / HR (h = 0.025) pairs, 2,000 per geometry on
+NACA0012 and NACA2412, (M, alpha) drawn without repetition by
+default_rng(1) from the same grid, HLLC + MUSCL; one network per geometry;
+then 64 held-out cases per geometry (drawn by the same generator, none of
+them a training case) solved to their reference-field thresholds from the
+network's warm start, from IDW alone and from freestream: steps and wall
+time of each.
 """
 from __future__ import annotations
 
@@ -339,6 +349,93 @@ def small_config(meshes, starter, args):
     return out
 
 
+LEARNED_GEOMETRIES = ("naca0012", "naca2412")
+
+
+def learned_grid(n_train, n_test, seed=1):
+    """(train, test) lists of (M, alpha) per geometry, drawn without
+    repetition from M = 0.70 + 0.01 i (i = 0..230) x alpha = -5 + 0.5 j
+    (j = 0..20) by default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for g in LEARNED_GEOMETRIES:
+        pick = rng.choice(231 * 21, size=n_train + n_test, replace=False)
+        pts = [(round(0.70 + 0.01 * int(q // 21), 2), -5.0 + 0.5 * int(q % 21)) for q in pick]
+        out[g] = (pts[:n_train], pts[n_train:])
+    return out
+
+
+def learned_config(args):
+    """The learned warm start on its own training configuration (see the
+    module docstring); returns the JSON-able result."""
+    import torch
+    from eulersr_amd import pipeline as P
+    n_train, n_test, chunk = args.learned_pairs // len(LEARNED_GEOMETRIES), 64, 500
+    grids = learned_grid(n_train, n_test)
+    starter = P.WarmStarter()
+    training, cases = {}, []
+    for g in LEARNED_GEOMETRIES:
+        from eulersr_amd.meshing import build_mesh
+        lm, hm = build_mesh(g, args.h_lr), build_mesh(g, args.h)
+        train, test = grids[g]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        parts = []
+        for i in range(0, len(train), chunk):
+            parts.append(P.training_pairs(g, lm, hm, train[i:i + chunk]))
+            log(f"  {g}: {min(i + chunk, len(train))}/{len(train)} LR/HR pairs, {time.perf_counter() - t0:.0f}s")
+        pairs = dict(lr=torch.cat([q["lr"] for q in parts]), hr=torch.cat([q["hr"] for q in parts]),
+                     machs=np.concatenate([q["machs"] for q in parts]),
+                     aoas=np.concatenate([q["aoas"] for q in parts]))
+        del parts
+        torch.cuda.synchronize()
+        t_data = time.perf_counter() - t0
+        kmap = P.WS.KnnMap(lm, hm)
+        net, info = P.train(kmap, pairs, steps=args.train_steps, batch=args.train_batch, seed=0)
+        info.update(data_s=t_data, pairs=len(train), lr_cells=lm.n_cells, hr_cells=hm.n_cells)
+        training[g] = info
+        log(f"  {g}: {len(train)} pairs in {t_data:.0f}s; trained {info['steps']} steps on {info['rows']} rows in "
+            f"{info['train_s']:.1f}s, loss {info['loss_first']:.3e} -> {info['loss_last']:.3e} "
+            f"(IDW-only {info['idw_mse']:.3e})")
+        del pairs
+        torch.cuda.empty_cache()
+        starter.add(g, lm, hm, net=net, kmap=kmap)
+        hs, hi = P.make_cases(hm, g, test)
+        ref, _ = P.fixed_time(hs, hi)
+        cases += P.warm_cases(g, lm, hm, test, ref)
+    starter.campaign(cases[:4], mode="mlp", slots=4, max_steps=20, lr_max_steps=20)
+    runs = {}
+    for mode in ("mlp", "idw", "cold"):
+        r = starter.campaign(cases, mode=mode, slots=args.slots, max_steps=args.max_steps)
+        st = [x["stopping_step"] for x in r["reports"] if x["converged"]]
+        runs[mode] = dict(total_s=r["total_s"], hr_s=r["hr_s"], lr_s=r["lr_s"], inference_s=r["infer_s"],
+                          converged=int(sum(x["converged"] for x in r["reports"])),
+                          fallback=int(sum(r["fallback"])), steps_median=float(np.median(st)) if st else None,
+                          steps_mean=float(np.mean(st)) if st else None, cases_per_hour=rate(r),
+                          steps=[x["stopping_step"] for x in r["reports"]],
+                          cd=[x["cd"] for x in r["reports"]])
+        log(f"{mode}: {r['total_s']:.2f}s (HR {r['hr_s']:.2f}s), converged {runs[mode]['converged']}/{len(cases)}, "
+            f"steps median {runs[mode]['steps_median']} mean {runs[mode]['steps_mean']}, "
+            f"fallback {runs[mode]['fallback']}")
+    dcd = [abs(a - b) for a, b in zip(runs["mlp"]["cd"], runs["cold"]["cd"])]
+    return {
+        "metric": "HR steps and wall time to the reference-field threshold, learned warm start vs IDW and "
+                  "freestream starts",
+        "unit": "steps", "n_gpus": 1, "dtype": "f32 solver, bf16 network", "data": "synthetic",
+        "config": {"workload": f"{args.learned_pairs} solver-generated LR h={args.h_lr} / HR h={args.h} pairs, "
+                               f"{n_train} per geometry on NACA0012 and NACA2412, (M, aoa) by default_rng(1) "
+                               "without repetition from M in [0.7,3.0] x aoa in [-5,5]; HLLC+MUSCL, SSP-RK2; "
+                               f"{n_test} held-out cases per geometry solved with {args.slots} slots",
+                   "pairs": args.learned_pairs, "test_cases": len(cases)},
+        "training": training,
+        "runs": {m: {k: v for k, v in r.items() if k not in ("steps", "cd")} for m, r in runs.items()},
+        "steps_ratio_cold_over_mlp": (runs["cold"]["steps_mean"] / runs["mlp"]["steps_mean"]
+                                      if runs["mlp"]["steps_mean"] else None),
+        "hr_time_ratio_cold_over_mlp": runs["cold"]["hr_s"] / runs["mlp"]["hr_s"],
+        "abs_dcd_vs_cold": {"median": float(np.median(dcd)), "within_1e-4": int(sum(d <= 1e-4 for d in dcd))},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -362,6 +459,10 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3: every solve capped at 2000 steps, 200 training steps, "
                          "no single-case passes, no cold batch, no 6-case config, no CPU baseline")
+    ap.add_argument("--learned-config", action="store_true",
+                    help="measure the learned warm start on its training configuration (see the docstring) "
+                         "instead of the headline benchmark")
+    ap.add_argument("--learned-pairs", type=int, default=4000)
     args = ap.parse_args()
     if args.profile:
         args.max_steps = min(args.max_steps, 2000)
@@ -382,6 +483,9 @@ def main():
     if args.launch:
         p_, b_, x_ = args.launch.split(",")
         P.AUTOTUNE, P.LAUNCH = False, (p_, int(b_), bool(int(x_)))
+    if args.learned_config:
+        print(json.dumps(learned_config(args)), flush=True)
+        return 0
     log(f"building meshes h={args.h}, LR h={args.h_lr}")
     meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
     for g, (lm, hm) in meshes.items():
