@@ -165,7 +165,7 @@ typedef struct {
                                 the step's start state (the same dt in every stage;
                                 the global dt is its minimum over cells). A steady
                                 state is the same; the path to it, and therefore
-                                the stationarity residual, is not -- judge such
+                                the stationarity residual, is not -- measure such
                                 solves by CD/CL (engineering targets). */
   int32_t no_graph;          /* 1: launch every kernel eagerly instead of replaying
                                 a captured HIP graph of check_every steps */
