@@ -26,7 +26,9 @@ inference on the GPU, HR states with the freestream fallback, then the HR
 solves with continuous batching -- about 64 cases in flight in launch
 sub-groups of 6 cases of one kind, each slot refilled with the next waiting
 case of its kind as soon as its case is done (esr_solver_replace_case), idle
-sub-groups handed to the kinds with waiting cases (pipeline.campaign).
+sub-groups handed to the kinds with waiting cases, and, once nothing waits,
+free slots running the freestream fallback of warm starts still in flight
+beside them (pipeline.campaign).
 ``value`` = converged cases per hour over the K timed campaigns. Reported beside it: the plain single-case GPU solver (16 of
 the cases alone, cold start, same thresholds; its per-case rate), |dCD| of
 the warm start against those plain solves, the same campaign from cold

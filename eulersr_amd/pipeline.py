@@ -239,7 +239,8 @@ def initial_slots(specs, slots):
     return queues, first
 
 
-def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL, trace=None):
+def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL, trace=None,
+             hedge=True):
     """Continuous batching: solves every case of `specs` (initial states
     `inits`) to its own stationarity criterion with about `slots` cases in
     flight; whenever a case is done (converged, diverged or at its step cap
@@ -261,6 +262,15 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     restarted in its slot from fallback_inits[i] (freestream). With the K = 0
     limiter a warm start can settle on a limit cycle whose residual stays
     above the threshold set by the cold trajectory.
+
+    hedge (warm starts): once no case waits in any queue, a slot that falls
+    free in a sub-group still launched runs the fallback of a warm-started
+    case of its kind in flight (the one that has run longest, not yet
+    restarted or hedged), beside the warm start. The first of the two to
+    converge is the case's result (a hedge that wins counts as a restart),
+    and the other slot is retired; a warm start that stalls or diverges
+    while its hedge runs is retired in favour of the hedge. The free lanes of
+    the campaign's tail would otherwise idle in launches that run anyway.
 
     trace: a list that receives (seconds, step, cases in flight, cases
     waiting) after every chunk.
@@ -329,11 +339,40 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
         hist = {ci: [] for ci in first}
         restarted = []
         changed = False
+        slot_hedge = [None] * len(slot_case)   # case id whose fallback runs in the slot
+        hedge_of = {}                          # case id -> slot of its hedge
+
+        def retire(j, ci):
+            # one step of a placeholder (then done) ends whatever ran in slot j
+            sp = copy.copy(specs[ci])
+            sp.max_steps = 1
+            bs.replace_case(j, sp, fallback_inits[ci])
+
         while any(ci is not None for ci in slot_case) or any(queues.values()):
             # runs end on multiples of `chunk` (a multiple of the check
             # period), so replaced cases keep the check cadence of a solo run
             step = bs.run((step // chunk + 1) * chunk)
             reps = bs.reports()
+            for j, ci in enumerate(slot_hedge):
+                if ci is None:
+                    continue
+                r = reps[j]
+                p = slot_case.index(ci)
+                if r["converged"] and not reps[p]["converged"]:
+                    # the hedge wins: the warm start's slot is retired
+                    retire(p, ci)
+                    slot_case[p] = None
+                    results[ci] = r
+                    restarted.append(ci)
+                    slot_hedge[j] = None
+                    del hedge_of[ci]
+                    changed = True
+                elif r["converged"] or r["diverged"] or r["steps_run"] >= r["N"]:
+                    # done without winning (a finished copy stops computing):
+                    # the warm start converged in the same chunk and is kept,
+                    # or the hedge failed and the warm start goes on
+                    slot_hedge[j] = None
+                    del hedge_of[ci]
             for j, ci in enumerate(slot_case):
                 if ci is None:
                     continue
@@ -343,12 +382,25 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     if r["last_rel"] == r["last_rel"] and r["last_rel"] != float("inf"):
                         hist[ci].append((r["steps_run"], r["last_rel"]))
                     if r["diverged"] or (not done and stalled(hist[ci], r["steps_run"], *stall)):
-                        bs.replace_case(j, specs[ci], fallback_inits[ci])
+                        if ci in hedge_of:
+                            # its fallback already runs: that slot carries the case on
+                            h = hedge_of.pop(ci)
+                            retire(j, ci)
+                            slot_case[j] = None
+                            slot_hedge[h] = None
+                            slot_case[h] = ci
+                            changed = True
+                        else:
+                            bs.replace_case(j, specs[ci], fallback_inits[ci])
                         restarted.append(ci)
                         continue
                 if not done:
                     continue
                 results[ci] = r
+                if ci in hedge_of:
+                    h = hedge_of.pop(ci)
+                    retire(h, ci)
+                    slot_hedge[h] = None
                 q = queues[_kind(specs[ci])]
                 if q:
                     nxt = q.popleft()
@@ -361,11 +413,13 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
             # open idle sub-groups of the kinds with the most waiting cases
             # (a live sub-group costs all of its slots in every launch, so the
             # launched slots are bounded as well as the cases in flight)
+            def busy(j):
+                return slot_case[j] is not None or slot_hedge[j] is not None
             active = sum(ci is not None for ci in slot_case)
-            lanes = S * sum(any(slot_case[j] is not None for j in sl) for sl in sub_slots)
+            lanes = S * sum(any(busy(j) for j in sl) for sl in sub_slots)
             while active <= slots - S and lanes + S <= LANE_FACTOR * slots:
                 waiting = [(len(queues[sub_kind[g]]), g) for g in range(len(sub_kind))
-                           if queues[sub_kind[g]] and all(slot_case[j] is None for j in sub_slots[g])]
+                           if queues[sub_kind[g]] and not any(busy(j) for j in sub_slots[g])]
                 if not waiting:
                     break
                 g = max(waiting, key=lambda x: (x[0], -x[1]))[1]
@@ -380,6 +434,23 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     active += 1
                 lanes += S
                 changed = True
+            # hedges: free slots of launched sub-groups, once nothing waits
+            if hedge and fallback_inits is not None and not any(queues.values()):
+                for g, sl in enumerate(sub_slots):
+                    if not any(slot_case[j] is not None for j in sl):
+                        continue
+                    for j in sl:
+                        if busy(j):
+                            continue
+                        cand = [(reps[p]["steps_run"], ci) for p, ci in enumerate(slot_case)
+                                if ci is not None and ci not in restarted and ci not in hedge_of
+                                and _kind(specs[ci]) == sub_kind[g]]
+                        if not cand:
+                            break
+                        ci = max(cand)[1]
+                        bs.replace_case(j, specs[ci], fallback_inits[ci])
+                        slot_hedge[j] = ci
+                        hedge_of[ci] = j
             if changed:
                 bs.compact()        # sub-groups with no case left stop launching
                 changed = False

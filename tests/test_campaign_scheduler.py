@@ -164,3 +164,58 @@ def test_chunk_must_follow_the_check_period(mock):
     specs, inits = _cases(1, 2, lambda i, k: 100)
     with pytest.raises(ValueError):
         P.campaign(specs, inits, slots=2, chunk=42)
+
+
+def test_hedge_finishes_a_stuck_warm_start_once_nothing_waits(mock):
+    # case 5 (the last to start) never converges from its warm start and the
+    # stall rule is out of reach: once the queue is empty a free slot runs its
+    # fallback beside it, and the fallback's result is the case's
+    specs, inits = _cases(1, 6, lambda i, k: 1500)
+    inits[5] = "stuck"
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=3, chunk=100, fallback_inits=fb,
+                                    stall=(10 ** 8, 500, 0.97))
+    assert restarted == [5]
+    assert all(r is not None and r["converged"] for r in reps)
+    assert reps[5]["stopping_step"] == specs[5].tag["cold"]
+    bs = mock.instances[0]
+    hedges = [x for x in bs.log if x[0] == "replace" and x[2] == 5]
+    assert len(hedges) >= 2                      # its warm start, then its hedge (and the retirement)
+    # the hedge started only after the last queued case had been placed
+    last_refill = max(x[3] for x in bs.log if x[0] == "replace" and x[2] != 5)
+    first_hedge = [x[3] for x in hedges][1]
+    assert first_hedge >= last_refill
+
+
+def test_warm_start_that_converges_first_keeps_its_result(mock):
+    # a slow but converging warm start beats its hedge (cold cost 2x)
+    specs, inits = _cases(1, 4, lambda i, k: 1000 if i != 3 else 1800)
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=2, chunk=100, fallback_inits=fb)
+    assert restarted == []
+    assert reps[3]["converged"] and reps[3]["stopping_step"] == 1800
+
+
+def test_no_hedges_without_fallback_or_when_off(mock):
+    specs, inits = _cases(1, 4, lambda i, k: 1000 if i != 3 else 1800)
+    for kw in (dict(), dict(fallback_inits=["fallback"] * 4, hedge=False)):
+        MockSolver.instances = []
+        reps, t, restarted = P.campaign(specs, inits, slots=2, chunk=100, **kw)
+        assert restarted == [] and all(r["converged"] for r in reps)
+        bs = MockSolver.instances[0]
+        assert not [x for x in bs.log if x[0] == "replace" and x[2] == 3 and x[3] > 1000]
+
+
+def test_stall_with_a_running_hedge_hands_the_case_to_the_hedge(mock):
+    specs, inits = _cases(1, 4, lambda i, k: 1000)
+    inits[3] = "stuck"
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=2, chunk=100, fallback_inits=fb,
+                                    stall=(1500, 500, 0.97))
+    assert restarted == [3]
+    assert reps[3]["converged"] and reps[3]["stopping_step"] == specs[3].tag["cold"]
+    bs = mock.instances[0]
+    # the stuck warm start was not restarted in its own slot beside the hedge:
+    # its case id was placed twice (warm start, hedge) plus one retirement
+    placed = [x for x in bs.log if x[0] == "replace" and x[2] == 3]
+    assert len(placed) == 3
