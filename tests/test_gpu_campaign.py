@@ -74,3 +74,46 @@ def test_replacement_of_another_kind_is_refused():
     bs.run(20)
     assert bs.report(1)["steps_run"] == 10
     bs.close()
+
+
+def test_hedged_campaign_matches_solo_runs_from_the_winning_start():
+    """Warm starts (a perturbed freestream) with freestream fallbacks in 4
+    slots (sub-groups of 2): once nothing waits, free slots run fallbacks
+    beside the warm starts still in flight. Every case's result equals, bit
+    for bit, a solo run from the start that produced it: its warm start, or
+    freestream for a case that was restarted or whose hedge won."""
+    from conftest import perturbed_state
+    from eulersr_amd import pipeline as P
+    from eulersr_amd.solve import BatchSolver
+    specs, cold = _cases()
+    warm = []
+    for i, sp in enumerate(specs):
+        cfg = base_cfg(Mach=float(np.hypot(*sp.kw["value"][1:3]) / sp.kw["value"][0] / np.sqrt(1.4)))
+        W, _ = perturbed_state(sp.mesh, dict(cfg, aoa=0.0), amp=0.03, seed=i)
+        warm.append(W)
+    # a poor prediction for case 4 (NACA0012, M = 1.8): the subsonic
+    # freestream of M = 0.9; its hedge from freestream should finish first
+    from eulersr_amd import main as M
+    warm[4], _ = M.initialize(specs[4].mesh, base_cfg(Mach=0.9, aoa=0.0, case="naca0012"))
+    old = (P.AUTOTUNE, P.LAUNCH)
+    P.AUTOTUNE, P.LAUNCH = False, LAUNCH
+    try:
+        reps, t, restarted = P.campaign(specs, warm, slots=4, max_steps=3000, chunk=40, fallback_inits=cold)
+    finally:
+        P.AUTOTUNE, P.LAUNCH = old
+    print(f"restarted {restarted}", flush=True)
+    assert 4 in restarted
+    for i, sp in enumerate(specs):
+        sp1 = copy.copy(sp)
+        sp1.max_steps = 3000
+        one = BatchSolver([sp1], threshold=1.0, check_every=5, patience=3, launch=LAUNCH)
+        one.set_state(0, cold[i] if i in restarted else warm[i])
+        one.prepare()
+        one.run()
+        r = one.report(0)
+        one.close()
+        c = reps[i]
+        print(f"case {i}: campaign steps {c['stopping_step']} conv {c['converged']}; alone steps "
+              f"{r['stopping_step']} conv {r['converged']}", flush=True)
+        assert c["converged"] == r["converged"] and c["stopping_step"] == r["stopping_step"]
+        assert c["cd"] == r["cd"] and c["cl"] == r["cl"]
