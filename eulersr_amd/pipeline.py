@@ -337,6 +337,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
         if AUTOTUNE and LAUNCH is None:
             bs.tune()
         hist = {ci: [] for ci in first}
+        placed = {ci: step for ci in first}    # step at which each case entered its slot
         restarted = []
         changed = False
         slot_hedge = [None] * len(slot_case)   # case id whose fallback runs in the slot
@@ -407,6 +408,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     bs.replace_case(j, specs[nxt], inits[nxt])
                     slot_case[j] = nxt
                     hist[nxt] = []
+                    placed[nxt] = step
                 else:
                     slot_case[j] = None
                     changed = True
@@ -431,6 +433,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     bs.replace_case(j, specs[nxt], inits[nxt])
                     slot_case[j] = nxt
                     hist[nxt] = []
+                    placed[nxt] = step
                     active += 1
                 lanes += S
                 changed = True
@@ -442,7 +445,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     for j in sl:
                         if busy(j):
                             continue
-                        cand = [(reps[p]["steps_run"], ci) for p, ci in enumerate(slot_case)
+                        cand = [(step - placed[ci], ci) for ci in slot_case
                                 if ci is not None and ci not in restarted and ci not in hedge_of
                                 and _kind(specs[ci]) == sub_kind[g]]
                         if not cand:

@@ -219,3 +219,17 @@ def test_stall_with_a_running_hedge_hands_the_case_to_the_hedge(mock):
     # its case id was placed twice (warm start, hedge) plus one retirement
     placed = [x for x in bs.log if x[0] == "replace" and x[2] == 3]
     assert len(placed) == 3
+
+
+def test_hedge_goes_to_the_case_longest_in_its_slot(mock):
+    # cases 0 and 1 finish together while case 3 (the last in the queue)
+    # takes one of their slots: the other free slot hedges case 2, which has
+    # run since the start, not case 3, which has just entered
+    costs = {0: 2000, 1: 2000, 2: 9000, 3: 9000}
+    specs, inits = _cases(1, 4, lambda i, k: costs[i])
+    fb = ["fallback"] * len(specs)
+    P.campaign(specs, inits, slots=3, chunk=100, fallback_inits=fb)
+    bs = mock.instances[0]
+    later = [x for x in bs.log if x[0] == "replace" and x[3] >= 2000]
+    assert later[0][2] == 3                  # the refill
+    assert later[1][2] == 2                  # the first hedge
