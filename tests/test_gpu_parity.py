@@ -138,21 +138,23 @@ def test_forces_match_oracle():
     assert abs(cl - rcl) <= 1e-6 * sy
 
 
-def test_100_steps_primitives_within_1e5():
-    """per-cell primitives within max relative 1e-5 after 100
-    SRK2 steps (HLLC + MUSCL), relative to max(|p|, freestream scale)."""
+@pytest.mark.parametrize("flux", ["hllc", "roe"])
+def test_100_steps_primitives_within_1e5(flux):
+    """The per-cell bar: primitives within max relative 1e-5 after 100 SRK2
+    steps (MUSCL, the two fluxes of the batched campaigns), relative to
+    max(|p|, freestream scale)."""
     from eulersr_amd import euler_solver
     F = _oracle()
     mesh = mesh_of("diamond", 0.1)
     cfg = base_cfg(Mach=2.5, aoa=5.0)
     W0, inlet = F.initialize(mesh.n_cells, cfg, np.float32)
-    kw = _kw(inlet, "hllc", "muscl")
+    kw = _kw(inlet, flux, "muscl")
     om = F.omesh(mesh, np.float32)
     dt = F.get_dt(W0, om, CFL=0.6)
     import torch
     Wg = torch.from_numpy(W0).cuda()
     Wr = W0.copy()
-    print("case: 100 SRK2 steps diamond h=0.1", flush=True)
+    print(f"case: 100 SRK2 steps diamond h=0.1 {flux}", flush=True)
     for _ in range(100):
         Wg = euler_solver.time_step_RK2_SSP(Wg, mesh, dt, **kw)
         Wr = F.time_step_RK2_SSP(Wr, om, dt, **kw)
