@@ -349,6 +349,10 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
             sp.max_steps = 1
             bs.replace_case(j, sp, fallback_inits[ci])
 
+        def busy(j):
+            # a case, or the hedge of one, runs in slot j
+            return slot_case[j] is not None or slot_hedge[j] is not None
+
         while any(ci is not None for ci in slot_case) or any(queues.values()):
             # runs end on multiples of `chunk` (a multiple of the check
             # period), so replaced cases keep the check cadence of a solo run
@@ -415,8 +419,6 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
             # open idle sub-groups of the kinds with the most waiting cases
             # (a live sub-group costs all of its slots in every launch, so the
             # launched slots are bounded as well as the cases in flight)
-            def busy(j):
-                return slot_case[j] is not None or slot_hedge[j] is not None
             active = sum(ci is not None for ci in slot_case)
             lanes = S * sum(any(busy(j) for j in sl) for sl in sub_slots)
             while active <= slots - S and lanes + S <= LANE_FACTOR * slots:
