@@ -1,6 +1,6 @@
-"""The case grids of the benchmark (host logic, no GPU): the ragged batch of
-the headline campaign and the training / held-out split of the learned
-warm-start configuration."""
+"""Host logic of the benchmark (no GPU): the case grids of the headline
+campaign and of the learned warm-start configuration, the campaign's slot
+occupancy from its trace, and the lookup of profiled HBM traffic."""
 import sys
 from pathlib import Path
 
@@ -30,3 +30,27 @@ def test_ragged_grid_stays_off_the_training_grid():
     for g, m, a, f in grid:
         assert 0.7 <= m <= 3.01 and a in AOAS
         assert not (m in bench.TRAIN_MACHS and a in bench.TRAIN_AOAS)
+
+
+def test_occupancy_from_a_campaign_trace():
+    # (seconds, step, cases in flight, cases waiting) after every chunk
+    trace = [(0.0, 100, 4, 6), (1.0, 200, 4, 2), (2.0, 300, 4, 0), (3.0, 400, 2, 0), (4.0, 500, 1, 0)]
+    occ = bench.occupancy(trace, slots=4)
+    # in flight over each interval, weighted by its length: (4 + 4 + 2 + 1) / 4
+    assert occ["mean_cases_in_flight"] == 11 / 4
+    assert occ["queue_empty_at_s"] == 2.0 and occ["tail_s"] == 2.0 and occ["hr_s"] == 4.0
+    assert bench.occupancy(trace[:1], slots=4) is None
+
+
+def test_measured_traffic_needs_the_profiled_launch(tmp_path, monkeypatch):
+    path = tmp_path / "traffic_bench_x.txt"
+    path.write_text("# launch cell,64,1\n# kernel fetch write\nk_cell<1, 100, 0> 100 20\nk_cell<1, 4, 0> 10 2\n"
+                    "k_recon 50 80\nk_update<2> 30 10\n")
+    monkeypatch.setattr(bench, "traffic_file", lambda tag: str(path))
+    pinned = {"path": "cell", "block": 64, "xcd_order": True}
+    # flux-templated kernels: every instantiation of the launch set, summed
+    assert bench.measured_traffic("k_cell<stage1>", pinned, "x") == 132.0
+    assert bench.measured_traffic("k_recon", pinned, "x") == 130.0
+    assert bench.measured_traffic("k_update<stage1>", pinned, "x") == 40.0
+    # another launch configuration than the profiled one: no traffic figure
+    assert bench.measured_traffic("k_cell<stage1>", dict(pinned, block=128), "x") is None
