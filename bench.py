@@ -64,12 +64,13 @@ import time
 
 import numpy as np
 
+T_START = time.time()
 HBM_PEAK_GBS = 8000.0
 ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench {time.time() - T_START:6.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def recon_bytes(items, cells, f_items, faces):

@@ -28,6 +28,7 @@ class MeshDesc(C.Structure):
         ("barycenter", f32p), ("area", f32p), ("surface", f32p), ("normals", f32p),
         ("inlet_subsonic", f32p), ("n_wall_markers", C.c_int32), ("wall_markers", i32p),
         ("force_marker", C.c_int32), ("sponge", C.c_int32), ("renumber", C.c_int32),
+        ("rcm_seed_order", i32p),
     ]
 
 
@@ -82,7 +83,7 @@ SIGNATURES = {
     "esr_mesh_destroy": (C.c_int, [C.c_void_p]),
     "esr_mesh_permutation": (C.c_int, [C.c_void_p, i32p]),
     "esr_mesh_info": (C.c_int, [C.c_void_p, i32p, i32p, C.POINTER(C.c_int64)]),
-    "esr_rcm": (C.c_int, [C.c_int32, i32p, i32p]),
+    "esr_rcm": (C.c_int, [C.c_int32, i32p, i32p, i32p]),
     "esr_residual": (C.c_int, [C.c_void_p, C.POINTER(Opts), C.c_void_p, C.c_void_p, C.c_void_p]),
     "esr_step": (C.c_int, [C.c_void_p, C.POINTER(Opts), C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
                            C.c_void_p]),

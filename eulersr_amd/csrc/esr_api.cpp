@@ -652,12 +652,12 @@ esr_status esr_mesh_info(const esr_mesh* mesh, int32_t* n_cells, int32_t* n_wall
   });
 }
 
-esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, int32_t* perm) {
+esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, const int32_t* seed_order, int32_t* perm) {
   return guard([&] {
     if (n_cells <= 0 || !neighbors || !perm) throw Error(ESR_ERR_VALUE, "bad arguments");
     for (int64_t t = 0; t < 3 * (int64_t)n_cells; ++t)
       if (neighbors[t] < -1 || neighbors[t] >= n_cells) throw Error(ESR_ERR_VALUE, "neighbors out of range");
-    const std::vector<int32_t> p = rcm_order(n_cells, neighbors);
+    const std::vector<int32_t> p = rcm_order(n_cells, neighbors, seed_order);
     std::memcpy(perm, p.data(), p.size() * sizeof(int32_t));
   });
 }

@@ -242,7 +242,7 @@ struct esr_mesh {
 namespace esr {
 // esr_mesh.cpp
 esr_mesh* build_mesh(const esr_mesh_desc* d);
-std::vector<int32_t> rcm_order(int32_t n, const int32_t* neighbors);
+std::vector<int32_t> rcm_order(int32_t n, const int32_t* neighbors, const int32_t* seed_order);
 std::vector<float> sponge_weights(const esr_mesh* m, float width);
 void set_last_error(const std::string& s);
 const char* last_error_cstr();

@@ -23,10 +23,13 @@ thread_local std::string g_last_error;
 void set_last_error(const std::string& s) { g_last_error = s; }
 const char* last_error_cstr() { return g_last_error.c_str(); }
 
-// Reverse Cuthill-McKee over the cell adjacency graph. Seeds: lowest
-// degree, ties by index; neighbours appended by (degree, index); the final
+// Reverse Cuthill-McKee over the cell adjacency graph, the algorithm of
+// scipy.sparse.csgraph.reverse_cuthill_mckee: components are seeded in
+// `seed_order` (a permutation of the cells; NULL = by degree, ties by
+// index); breadth-first levels; the unvisited neighbours of each node are
+// appended in increasing index, then stably sorted by degree; the final
 // order is reversed. perm[new] = old.
-std::vector<int32_t> rcm_order(int32_t n, const int32_t* nb) {
+std::vector<int32_t> rcm_order(int32_t n, const int32_t* nb, const int32_t* seed_order) {
   std::vector<int32_t> deg(n, 0);
   std::vector<int32_t> adj(3 * (size_t)n, -1);
   for (int32_t i = 0; i < n; ++i) {
@@ -44,8 +47,18 @@ std::vector<int32_t> rcm_order(int32_t n, const int32_t* nb) {
     deg[i] = cnt;
   }
   std::vector<int32_t> seeds(n);
-  std::iota(seeds.begin(), seeds.end(), 0);
-  std::stable_sort(seeds.begin(), seeds.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+  if (seed_order) {
+    std::vector<char> seen(n, 0);
+    for (int32_t q = 0; q < n; ++q) {
+      const int32_t s = seed_order[q];
+      if (s < 0 || s >= n || seen[s]) throw Error(ESR_ERR_VALUE, "RCM seed order is not a permutation of the cells");
+      seen[s] = 1;
+      seeds[q] = s;
+    }
+  } else {
+    std::iota(seeds.begin(), seeds.end(), 0);
+    std::stable_sort(seeds.begin(), seeds.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+  }
   std::vector<char> vis(n, 0);
   std::vector<int32_t> order;
   order.reserve(n);
@@ -62,14 +75,13 @@ std::vector<int32_t> rcm_order(int32_t n, const int32_t* nb) {
         int32_t j = adj[3 * (size_t)i + q];
         if (!vis[j]) cand[cnt++] = j;
       }
-      std::stable_sort(cand, cand + cnt, [&](int32_t a, int32_t b) {
-        return deg[a] < deg[b] || (deg[a] == deg[b] && a < b);
-      });
+      std::stable_sort(cand, cand + cnt, [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
       for (int q = 0; q < cnt; ++q) {
         vis[cand[q]] = 1;
         order.push_back(cand[q]);
       }
     }
+    if ((int32_t)order.size() == n) break;
   }
   std::reverse(order.begin(), order.end());
   return order;
@@ -121,7 +133,7 @@ esr_mesh* build_mesh(const esr_mesh_desc* d) {
 
     // renumbering
     if (d->renumber) {
-      m->perm = rcm_order(n, NB);
+      m->perm = rcm_order(n, NB, d->rcm_seed_order);
     } else {
       m->perm.resize(n);
       std::iota(m->perm.begin(), m->perm.end(), 0);

@@ -11,6 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib as L
+from .mesh import rcm_seed_order
 
 _torch = None
 
@@ -110,6 +111,7 @@ class DeviceMesh:
         d.force_marker = int(md.get("force_marker", 2))
         d.sponge = 0 if md.get("case") == "bump" else 1
         d.renumber = 1 if renumber else 0
+        d.rcm_seed_order = i32(rcm_seed_order(mesh.neighbors)) if renumber else None
         h = C.c_void_p()
         L.check(L.lib().esr_mesh_create(C.byref(d), C.byref(h)), "esr_mesh_create")
         self.handle = h

@@ -40,6 +40,22 @@ def edge_keys(a: np.ndarray, b: np.ndarray, n_points: int) -> np.ndarray:
     return lo * np.int64(n_points) + hi
 
 
+def rcm_seed_order(neighbors) -> np.ndarray:
+    """Order in which reverse Cuthill-McKee seeds the components of the cell
+    graph: ``numpy.argsort`` of the node degrees (distinct neighbours, no self
+    loops), the call scipy.sparse.csgraph.reverse_cuthill_mckee makes. Among
+    cells of equal degree the default sort's order is numpy's choice, so
+    passing this order to the library's ordering reproduces scipy's
+    permutation on the machine that runs it."""
+    nb = np.asarray(neighbors, dtype=np.int64)
+    own = np.arange(nb.shape[0], dtype=np.int64)[:, None]
+    ok = (nb >= 0) & (nb != own)
+    srt = np.sort(np.where(ok, nb, -1), axis=1)
+    distinct = (srt >= 0) & np.concatenate([np.ones((nb.shape[0], 1), bool), srt[:, 1:] != srt[:, :-1]], axis=1)
+    deg = distinct.sum(axis=1).astype(np.int32)
+    return np.argsort(deg).astype(np.int32)
+
+
 def faces_neighbors_from_tris(tris: np.ndarray, n_points: int):
     """Unique edges in first-appearance order (triangle-major, local edge
     minor), neighbours across each local edge and a boundary mask.

@@ -75,6 +75,11 @@ typedef struct {
   int32_t force_marker;                     /* helper.py:370 (default 2) */
   int32_t sponge;                           /* 0 when metadata case=='bump' (helper.py:266) */
   int32_t renumber;                         /* 1: reverse Cuthill-McKee  */
+  const int32_t* rcm_seed_order;            /* (n_cells,) order in which RCM
+                                               seeds components, as
+                                               numpy.argsort(degree) in scipy's
+                                               reverse_cuthill_mckee; NULL: by
+                                               degree, ties by lowest index  */
 } esr_mesh_desc;
 
 /* Solver keyword arguments of residual()/time_step_*() (euler_solver.py:602-667,
@@ -101,9 +106,12 @@ esr_status esr_mesh_permutation(const esr_mesh* mesh, int32_t* perm);
 esr_status esr_mesh_info(const esr_mesh* mesh, int32_t* n_cells, int32_t* n_wall_faces,
                          int64_t* device_bytes);
 /* Host-only: the reverse Cuthill-McKee order esr_mesh_create uses, from a
- * (n_cells,3) neighbour table (-1 = boundary). perm[new] = old. No GPU needed.
- * No reference counterpart (the reference keeps the mesher's order). */
-esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, int32_t* perm);
+ * (n_cells,3) neighbour table (-1 = boundary); the algorithm of
+ * scipy.sparse.csgraph.reverse_cuthill_mckee (scipy 1.15.3, symmetric mode),
+ * seeding components in `seed_order` (NULL: by degree, ties by index).
+ * perm[new] = old. No GPU needed. No reference counterpart (the reference
+ * keeps the mesher's order). */
+esr_status esr_rcm(int32_t n_cells, const int32_t* neighbors, const int32_t* seed_order, int32_t* perm);
 
 /* -- single-state operations (device W, original cell order) -------------- */
 

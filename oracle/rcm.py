@@ -1,15 +1,24 @@
 """CPU oracle (TEST INFRASTRUCTURE ONLY): reverse Cuthill-McKee ordering of
-the cell adjacency graph, restated from the published algorithm (Cuthill &
-McKee 1969; George & Liu 1981, "reverse" variant):
+the cell adjacency graph.
 
-* nodes are visited component by component; each component starts from its
-  lowest-degree unvisited node (ties: lowest index);
-* breadth-first search; the unvisited neighbours of each node are appended in
-  increasing degree (ties: lowest index);
+The pin is scipy's ``scipy.sparse.csgraph.reverse_cuthill_mckee`` (scipy
+1.15.3, ``scipy/sparse/csgraph/_reordering.pyx``), whose published algorithm
+this file restates step for step in plain Python:
+
+* node degree = stored entries of the CSR row, plus one more when the row
+  holds its diagonal (``_node_degrees``);
+* components are seeded in the order ``numpy.argsort(degree)`` (the default,
+  unstable sort: which of several minimum-degree nodes comes first is the
+  sort's choice, and differs between numpy's SIMD dispatch targets); the
+  caller may give that seed order explicitly;
+* breadth-first search level by level; for each node the unvisited
+  neighbours are appended in CSR order, then insertion-sorted by degree
+  (stable: equal degrees keep CSR order);
 * the final order is reversed.
 
-The product's C++ ordering (eulersr_amd/csrc/esr_mesh.cpp) must equal this one
-element for element. The reference has no renumbering (the new layout is ours).
+The product's C++ ordering (eulersr_amd/csrc/esr_mesh.cpp) must equal this
+one element for element for the same seed order. The reference has no
+renumbering (the new layout is ours).
 """
 from __future__ import annotations
 
@@ -17,34 +26,80 @@ import numpy as np
 
 
 def cell_graph(neighbors):
-    """Adjacency lists (sorted) from a (N,3) neighbour table, -1 = none."""
-    n = len(neighbors)
-    adj = [sorted({int(j) for j in neighbors[i] if j >= 0 and j != i}) for i in range(n)]
-    return adj
+    """CSR (indptr, indices) of the cell adjacency graph of a (N,3)
+    neighbour table (-1 = none): sorted, duplicate-free, no self loops -- the
+    canonical CSR scipy builds from these pairs."""
+    nb = np.asarray(neighbors)
+    n = len(nb)
+    rows = [sorted({int(j) for j in nb[i] if j >= 0 and j != i}) for i in range(n)]
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    indptr[1:] = np.cumsum([len(r) for r in rows])
+    indices = np.array([j for r in rows for j in r], dtype=np.int64)
+    return indptr, indices
 
 
-def rcm(neighbors):
-    adj = cell_graph(neighbors)
-    n = len(adj)
-    deg = np.array([len(a) for a in adj])
-    seeds = sorted(range(n), key=lambda i: (deg[i], i))
+def node_degrees(indptr, indices):
+    n = len(indptr) - 1
+    deg = np.zeros(n, dtype=np.int32)
+    for i in range(n):
+        deg[i] = indptr[i + 1] - indptr[i]
+        for jj in range(indptr[i], indptr[i + 1]):
+            if indices[jj] == i:
+                deg[i] += 1
+                break
+    return deg
+
+
+def seed_order(neighbors):
+    """The component seed order scipy uses for this graph on this machine."""
+    indptr, indices = cell_graph(neighbors)
+    return np.argsort(node_degrees(indptr, indices))
+
+
+def rcm_csr(indptr, indices, seeds=None):
+    """Reverse Cuthill-McKee of a symmetric CSR graph; ``seeds`` = component
+    seed order (default ``numpy.argsort`` of the degrees). Returns perm with
+    perm[new] = old."""
+    n = len(indptr) - 1
+    deg = node_degrees(indptr, indices)
+    seeds = np.argsort(deg) if seeds is None else np.asarray(seeds)
+    assert sorted(int(s) for s in seeds) == list(range(n)), "seed order must be a permutation"
     visited = np.zeros(n, dtype=bool)
     order = []
     for s in seeds:
+        s = int(s)
         if visited[s]:
             continue
         visited[s] = True
         order.append(s)
-        head = len(order) - 1
-        while head < len(order):
-            i = order[head]
-            head += 1
-            nb = [j for j in adj[i] if not visited[j]]
-            nb.sort(key=lambda j: (deg[j], j))
-            for j in nb:
-                visited[j] = True
-                order.append(j)
+        lo, hi = len(order) - 1, len(order)
+        while lo < hi:
+            for ii in range(lo, hi):
+                i = order[ii]
+                start = len(order)
+                for jj in range(indptr[i], indptr[i + 1]):
+                    j = int(indices[jj])
+                    if not visited[j]:
+                        visited[j] = True
+                        order.append(j)
+                # insertion sort of the new nodes by degree (stable)
+                for kk in range(start + 1, len(order)):
+                    v = order[kk]
+                    ll = kk
+                    while ll > start and deg[v] < deg[order[ll - 1]]:
+                        order[ll] = order[ll - 1]
+                        ll -= 1
+                    order[ll] = v
+            lo, hi = hi, len(order)
+        if len(order) == n:
+            break
     return np.array(order[::-1], dtype=np.int32)
+
+
+def rcm(neighbors, seeds=None):
+    """RCM of the cell adjacency graph of a (N,3) neighbour table."""
+    indptr, indices = cell_graph(neighbors)
+    return rcm_csr(indptr, indices, seeds)
 
 
 def bandwidth(neighbors, perm):

@@ -12,7 +12,8 @@ Contents:
              as built by eulersr_amd.meshing.build_mesh("diamond", 0.3)
   geo_*      barycenter, area, surface, normals, face_connectivity derived
              from them (mesh.py:223-260 conventions)
-  rcm_perm   reverse Cuthill-McKee order of the cell graph (oracle/rcm.py)
+  rcm_perm   reverse Cuthill-McKee order of the cell graph (oracle/rcm.py),
+             seeded at the lowest-index cell of least degree
   W, inlet   a seeded smooth perturbation of the M = 1.8, alpha = 5 deg
              freestream (float32 conserved) and the inlet state
   R64_<flux>_<recon>, R32_<flux>_<recon>
@@ -70,7 +71,8 @@ def main():
     out = {f"mesh_{k}": np.asarray(getattr(mesh, k)) for k in MESH_FIELDS}
     out["mesh_metadata"] = np.frombuffer(json.dumps(mesh.metadata).encode(), dtype=np.uint8)
     out.update({f"geo_{k}": np.asarray(getattr(mesh, k)) for k in GEO_FIELDS})
-    out["rcm_perm"] = np.asarray(RCM.rcm(mesh.neighbors), dtype=np.int32)
+    stable = np.argsort((np.asarray(mesh.neighbors) >= 0).sum(axis=1), kind="stable")
+    out["rcm_perm"] = np.asarray(RCM.rcm(mesh.neighbors, stable), dtype=np.int32)
     out["W"], out["inlet"] = W, inlet
     for flux in FLUXES:
         for recon in RECONS:

@@ -36,7 +36,7 @@ def test_version_and_error_string():
     L = _lib.lib()
     assert L.esr_version() >= 100
     # a host-only call that fails sets the error string
-    st = L.esr_rcm(0, None, None)
+    st = L.esr_rcm(0, None, None, None)
     assert st == _lib.ESR_ERR_VALUE
     assert b"bad arguments" in L.esr_last_error()
 

@@ -1,9 +1,11 @@
 """Mesh layer: connectivity and geometry bit-exact against the loop-based
 restatement of the reference conventions (oracle/mesh_ref.py), hand-checked
-small meshes, the C++ reverse Cuthill-McKee order bit-exact against the
-oracle's, and mesher quality / cell-count pins."""
+small meshes, the C++ reverse Cuthill-McKee order equal to scipy's
+reverse_cuthill_mckee (live and against committed permutations), and mesher
+quality / cell-count pins."""
 import ctypes as C
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -88,38 +90,115 @@ def test_mesher_quality_and_published_cell_counts():
     assert mn.min() >= 28.0
 
 
-def test_rcm_cpp_equals_oracle_and_matches_scipy_bandwidth():
+def _esr_rcm(nb, seeds=None):
     from eulersr_amd import _lib as L
-    from scipy.sparse import csr_matrix
-    from scipy.sparse.csgraph import reverse_cuthill_mckee
-    m = mesh_of("naca0012", 0.2)
-    nb = np.ascontiguousarray(m.neighbors, dtype=np.int32)
-    perm = np.empty(m.n_cells, dtype=np.int32)
-    st = L.lib().esr_rcm(m.n_cells, nb.ctypes.data_as(C.POINTER(C.c_int32)),
+    nb = np.ascontiguousarray(nb, dtype=np.int32)
+    perm = np.empty(len(nb), dtype=np.int32)
+    sp = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.int32)
+    st = L.lib().esr_rcm(len(nb), nb.ctypes.data_as(C.POINTER(C.c_int32)),
+                         None if sp is None else sp.ctypes.data_as(C.POINTER(C.c_int32)),
                          perm.ctypes.data_as(C.POINTER(C.c_int32)))
     assert st == 0
-    ref = RCM.rcm(m.neighbors)
-    assert np.array_equal(perm, ref)
-    assert np.array_equal(np.sort(perm), np.arange(m.n_cells))
-    # same bandwidth class as scipy's reverse_cuthill_mckee on the cell graph
-    i = np.repeat(np.arange(m.n_cells), 3)
-    j = m.neighbors.reshape(-1)
+    return perm
+
+
+def _scipy_rcm(nb, diag=None):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    nb = np.asarray(nb)
+    n = len(nb)
+    i = np.repeat(np.arange(n), 3)
+    j = nb.reshape(-1)
     ok = j >= 0
-    A = csr_matrix((np.ones(ok.sum()), (i[ok], j[ok])), shape=(m.n_cells, m.n_cells))
-    sp = reverse_cuthill_mckee(A, symmetric_mode=True)
-    b_ours, b_sp, b_id = (RCM.bandwidth(m.neighbors, perm), RCM.bandwidth(m.neighbors, sp),
-                          RCM.bandwidth(m.neighbors, np.arange(m.n_cells)))
-    assert b_ours <= 1.5 * b_sp and b_ours < b_id
+    r, c = i[ok], j[ok]
+    if diag is not None:            # self loops on these nodes (degree + 2 in scipy's count)
+        r, c = np.concatenate([r, diag]), np.concatenate([c, diag])
+    A = csr_matrix((np.ones(len(r)), (r, c)), shape=(n, n))
+    return np.asarray(reverse_cuthill_mckee(A, symmetric_mode=True))
+
+
+RCM_MESHES = (("diamond", 0.1), ("diamond", 0.2), ("naca0012", 0.2), ("naca4412", 0.1), ("naca2412", 0.1))
+
+
+@pytest.mark.parametrize("geom,h", RCM_MESHES)
+def test_rcm_equals_scipy(geom, h):
+    """The library's ordering, given the seed order scipy takes from
+    numpy.argsort of the degrees (eulersr_amd.mesh.rcm_seed_order, what
+    DeviceMesh passes), equals scipy.sparse.csgraph.reverse_cuthill_mckee
+    element for element; so does the oracle's restatement."""
+    from eulersr_amd.mesh import rcm_seed_order
+    m = mesh_of(geom, h)
+    sp = _scipy_rcm(m.neighbors)
+    seeds = rcm_seed_order(m.neighbors)
+    assert np.array_equal(seeds, RCM.seed_order(m.neighbors))
+    assert np.array_equal(_esr_rcm(m.neighbors, seeds), sp)
+    assert np.array_equal(RCM.rcm(m.neighbors), sp)
+    assert RCM.bandwidth(m.neighbors, sp) < RCM.bandwidth(m.neighbors, np.arange(m.n_cells))
+
+
+@pytest.mark.parametrize("geom,h", RCM_MESHES)
+def test_rcm_default_seed_equals_scipy_from_that_seed(geom, h):
+    """Without a seed order the library seeds at the lowest-index cell of
+    least degree. scipy run on the same graph with self loops on every other
+    cell (which raises their degree by 2 in scipy's count, leaves the
+    neighbour order of every other cell unchanged, and so makes that cell
+    scipy's unique seed) gives the same permutation: the BFS, the degree
+    tie-break and the reversal are scipy's, whatever numpy's sort does."""
+    m = mesh_of(geom, h)
+    deg = (np.asarray(m.neighbors) >= 0).sum(axis=1)
+    s = int(np.argsort(deg, kind="stable")[0])
+    others = np.setdiff1d(np.arange(m.n_cells), [s])
+    sp = _scipy_rcm(m.neighbors, diag=others)
+    assert sp[-1] == s
+    assert np.array_equal(_esr_rcm(m.neighbors), sp)
+    assert np.array_equal(RCM.rcm(m.neighbors, np.argsort(deg, kind="stable")), sp)
+
+
+def test_rcm_golden_scipy_permutations():
+    """Permutations scipy computed (tests/golden/make_rcm.py), each with the
+    seed order it used: the library and the oracle reproduce them bit for
+    bit from the stored neighbour tables."""
+    gold = np.load(Path(__file__).parent / "golden" / "rcm_scipy.npz")
+    tags = sorted({k.rsplit("_", 1)[0] for k in gold.files})
+    assert len(tags) >= 5
+    for t in tags:
+        nb, seeds, perm = gold[f"{t}_neighbors"], gold[f"{t}_seeds"], gold[f"{t}_perm"]
+        assert np.array_equal(_esr_rcm(nb, seeds), perm), t
+        assert np.array_equal(RCM.rcm(nb, seeds), perm), t
+
+
+def test_rcm_oracle_equals_scipy_on_random_graphs():
+    """The oracle restates scipy's algorithm on general symmetric CSR graphs:
+    disconnected components, isolated nodes and self loops included."""
+    import scipy.sparse as sparse
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    rng = np.random.default_rng(7)
+    for _ in range(200):
+        n = int(rng.integers(1, 50))
+        k = int(rng.integers(0, 3 * n + 1))
+        i, j = rng.integers(0, n, k), rng.integers(0, n, k)
+        A = sparse.csr_matrix((np.ones(k), (i, j)), shape=(n, n))
+        A = ((A + A.T) > 0).astype(np.float64).tocsr()
+        A.sort_indices()
+        assert np.array_equal(RCM.rcm_csr(A.indptr, A.indices), reverse_cuthill_mckee(A, symmetric_mode=True))
+
+
+def test_rcm_rejects_bad_seed_order():
+    from eulersr_amd import _lib as L
+    nb = np.array([[1, -1, -1], [0, -1, -1]], dtype=np.int32)
+    perm = np.empty(2, dtype=np.int32)
+    bad = np.array([0, 0], dtype=np.int32)
+    st = L.lib().esr_rcm(2, nb.ctypes.data_as(C.POINTER(C.c_int32)), bad.ctypes.data_as(C.POINTER(C.c_int32)),
+                         perm.ctypes.data_as(C.POINTER(C.c_int32)))
+    assert st != 0
 
 
 def test_rcm_handles_disconnected_components():
     nb = np.array([[1, -1, -1], [0, -1, -1], [3, -1, -1], [2, -1, -1], [-1, -1, -1]], dtype=np.int32)
-    from eulersr_amd import _lib as L
-    perm = np.empty(5, dtype=np.int32)
-    assert L.lib().esr_rcm(5, nb.ctypes.data_as(C.POINTER(C.c_int32)),
-                           perm.ctypes.data_as(C.POINTER(C.c_int32))) == 0
-    assert np.array_equal(perm, RCM.rcm(nb))
+    perm = _esr_rcm(nb)
+    assert np.array_equal(perm, RCM.rcm(nb, np.argsort((nb >= 0).sum(axis=1), kind="stable")))
     assert sorted(perm.tolist()) == [0, 1, 2, 3, 4]
+    assert np.array_equal(_esr_rcm(nb, RCM.seed_order(nb)), _scipy_rcm(nb))
 
 
 def test_periodic_pairing():
