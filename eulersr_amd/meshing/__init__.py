@@ -93,9 +93,93 @@ def build_naca(name="naca0012", h=0.05, Lx=4.0, Ly=4.0, chord=1.0, cx=1.5, cy=2.
     return _assemble(P, tris, segs, smark, meta)
 
 
+BUMP, INLET, OUTLET, CHANNEL_WALL = 2, 3, 4, 6     # bump.py:12 marker codes
+
+
+def build_bump(h=0.05, Lx=3.0, Ly=1.0, thickness=0.04, chord=1.0, center=1.5, verbose=False) -> Mesh:
+    """Channel with a circular-arc bump on its lower wall (the bump case of
+    meshing/bump.py:139-210): inlet (3) on the left, outlet (4) on the right,
+    flat walls (6) below and above, the bump (2) between center -/+ chord/2.
+    Metadata case='bump' (no sponge, helper.py:266-267), wall markers [2, 6],
+    forces on the bump. Size law (ours): h within one chord of the bump,
+    growing linearly to 3h beyond."""
+    if thickness <= 0.0 or chord <= 0.0:
+        raise ValueError("the bump needs a positive thickness and chord")
+    if center - chord / 2.0 < 0.0 or center + chord / 2.0 > Lx:
+        raise ValueError("the bump must lie inside [0, Lx]")
+    R = ((chord / 2.0) ** 2 + thickness ** 2) / (2.0 * thickness)
+    yc = thickness - R
+    xl, xr = center - chord / 2.0, center + chord / 2.0
+
+    def size_many(pts):
+        pts = np.atleast_2d(pts)
+        dx = np.maximum(np.maximum(xl - pts[:, 0], pts[:, 0] - xr), 0.0)
+        d = np.hypot(dx, pts[:, 1])
+        return h * np.clip(1.0 + 2.0 * (d - 0.2 * chord) / chord, 1.0, 3.0)
+
+    def seg(p0, p1):
+        p0, p1 = np.asarray(p0, float), np.asarray(p1, float)
+        n = max(1, int(np.ceil(np.linalg.norm(p1 - p0) / float(size_many((p0 + p1) / 2)[0]))))
+        return p0 + (p1 - p0) * (np.arange(n) / n)[:, None]
+
+    n_arc = max(3, int(np.ceil(chord / h)))
+    xs = np.linspace(xl, xr, n_arc, endpoint=False)
+    arc = np.stack([xs, yc + np.sqrt(np.clip(R * R - (xs - center) ** 2, 0.0, None))], axis=1)
+    loops = [(seg((0.0, 0.0), (xl, 0.0)), CHANNEL_WALL), (arc, BUMP), (seg((xr, 0.0), (Lx, 0.0)), CHANNEL_WALL),
+             (seg((Lx, 0.0), (Lx, Ly)), OUTLET), (seg((Lx, Ly), (0.0, Ly)), CHANNEL_WALL),
+             (seg((0.0, Ly), (0.0, 0.0)), INLET)]
+    pts = np.concatenate([p for p, _ in loops])
+    marks = np.concatenate([np.full(len(p), m, dtype=np.int32) for p, m in loops])
+    n = len(pts)
+    segs = np.array([(i, (i + 1) % n) for i in range(n)])
+    # the region under the arc is a body touching the lower wall
+    body = np.concatenate([arc, [[xr, 0.0], [xr, -1.0], [xl, -1.0], [xl, 0.0]]])
+    dom = Domain(0.0, 0.0, Lx, Ly, [body])
+    P, tris, segs, smark = refine(pts, segs, marks, dom, size_many, verbose=verbose)
+    meta = dict(case="bump", h=h, domain={"Lx": Lx, "Ly": Ly}, center=center, thickness=thickness, chord=chord,
+                obstacle_length=chord, bump_marker=BUMP, force_marker=BUMP, wall_markers=[BUMP, CHANNEL_WALL],
+                inlet_marker=INLET, outlet_marker=OUTLET)
+    return _assemble(P, tris, segs, smark, meta)
+
+
+def build_periodic_channel(nx=40, ny=20, Lx=2.0, Ly=1.0, jitter=0.25, seed=0) -> Mesh:
+    """Channel periodic in y (marker 1 on the lower and upper sides, paired by
+    Mesh._set_periodic as mesh.py:266-321 pairs them), inlet (3) on the left,
+    outlet (4) on the right: a grid of nx x ny quads, each split along a
+    diagonal whose direction alternates, interior points moved by a seeded
+    jitter of up to `jitter` cells so the triangles are irregular."""
+    rng = np.random.default_rng(seed)
+    xs, ys = np.linspace(0.0, Lx, nx + 1), np.linspace(0.0, Ly, ny + 1)
+    X, Y = np.meshgrid(xs, ys)
+    P = np.stack([X.ravel(), Y.ravel()], axis=1)
+    inner = (X.ravel() > 0) & (X.ravel() < Lx) & (Y.ravel() > 0) & (Y.ravel() < Ly)
+    P[inner] += rng.uniform(-jitter, jitter, size=(int(inner.sum()), 2)) * np.array([Lx / nx, Ly / ny])
+    tris = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b = j * (nx + 1) + i, j * (nx + 1) + i + 1
+            c, d = (j + 1) * (nx + 1) + i + 1, (j + 1) * (nx + 1) + i
+            tris += [[a, b, c], [a, c, d]] if (i + j) % 2 == 0 else [[a, b, d], [b, c, d]]
+    tris = np.array(tris, dtype=np.int64)
+    faces, nb, bnd, _ = faces_neighbors_from_tris(tris, len(P))
+    fp = P[faces]
+    markers = np.zeros(len(faces), dtype=np.int32)
+    tol = 1e-12
+    on_x0 = bnd & (np.abs(fp[:, :, 0]) < tol).all(axis=1)
+    on_x1 = bnd & (np.abs(fp[:, :, 0] - Lx) < tol).all(axis=1)
+    markers[bnd] = 1
+    markers[on_x0] = INLET
+    markers[on_x1] = OUTLET
+    meta = dict(case="periodic_channel", domain={"Lx": Lx, "Ly": Ly}, wall_markers=[2], force_marker=2,
+                obstacle_length=1.0)
+    return Mesh.from_points(P, tris, nb, faces, markers, meta)
+
+
 def build_mesh(geometry: str, h: float, **kw) -> Mesh:
     if geometry == "diamond":
         return build_diamond(h=h, **kw)
+    if geometry == "bump":
+        return build_bump(h=h, **kw)
     if geometry in NACA:
         return build_naca(geometry, h=h, **kw)
     raise ValueError(f"unknown geometry '{geometry}', expected one of {GEOMETRIES}")

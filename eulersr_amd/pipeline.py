@@ -240,7 +240,7 @@ def initial_slots(specs, slots):
 
 
 def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=None, stall=STALL, trace=None,
-             hedge=True):
+             hedge=True, cold_ids=()):
     """Continuous batching: solves every case of `specs` (initial states
     `inits`) to its own stationarity criterion with about `slots` cases in
     flight; whenever a case is done (converged, diverged or at its step cap
@@ -272,8 +272,17 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     while its hedge runs is retired in favour of the hedge. The free lanes of
     the campaign's tail would otherwise idle in launches that run anyway.
 
+    cold_ids: cases whose initial state already is their fallback (a
+    non-physical prediction): they are neither restarted nor hedged, since
+    either would replay the same trajectory.
+
     trace: a list that receives (seconds, step, cases in flight, cases
     waiting) after every chunk.
+
+    Each report carries ``steps_total``: the steps the case took in every
+    slot it ran in -- a warm attempt that was restarted or lost to its hedge,
+    a hedge that lost, and the run that gave the result -- so step counts and
+    cell-updates of warm starts include the work they discarded.
 
     Returns (reports in input order, seconds of solving, restarted case ids)."""
     import math
@@ -339,6 +348,8 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
         hist = {ci: [] for ci in first}
         placed = {ci: step for ci in first}    # step at which each case entered its slot
         restarted = []
+        cold = set(cold_ids)
+        spent = {}                             # case id -> steps of its discarded runs
         changed = False
         slot_hedge = [None] * len(slot_case)   # case id whose fallback runs in the slot
         hedge_of = {}                          # case id -> slot of its hedge
@@ -367,7 +378,8 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     # the hedge wins: the warm start's slot is retired
                     retire(p, ci)
                     slot_case[p] = None
-                    results[ci] = r
+                    spent[ci] = spent.get(ci, 0) + reps[p]["steps_run"]
+                    results[ci] = dict(r, steps_total=spent[ci] + r["steps_run"])
                     restarted.append(ci)
                     slot_hedge[j] = None
                     del hedge_of[ci]
@@ -378,15 +390,17 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     # or the hedge failed and the warm start goes on
                     slot_hedge[j] = None
                     del hedge_of[ci]
+                    spent[ci] = spent.get(ci, 0) + r["steps_run"]
             for j, ci in enumerate(slot_case):
                 if ci is None:
                     continue
                 r = reps[j]
                 done = r["converged"] or r["diverged"] or r["steps_run"] >= r["N"]
-                if fallback_inits is not None and ci not in restarted and not r["converged"]:
+                if fallback_inits is not None and ci not in restarted and ci not in cold and not r["converged"]:
                     if r["last_rel"] == r["last_rel"] and r["last_rel"] != float("inf"):
                         hist[ci].append((r["steps_run"], r["last_rel"]))
                     if r["diverged"] or (not done and stalled(hist[ci], r["steps_run"], *stall)):
+                        spent[ci] = spent.get(ci, 0) + r["steps_run"]
                         if ci in hedge_of:
                             # its fallback already runs: that slot carries the case on
                             h = hedge_of.pop(ci)
@@ -401,11 +415,12 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                         continue
                 if not done:
                     continue
-                results[ci] = r
                 if ci in hedge_of:
                     h = hedge_of.pop(ci)
                     retire(h, ci)
                     slot_hedge[h] = None
+                    spent[ci] = spent.get(ci, 0) + reps[h]["steps_run"]
+                results[ci] = dict(r, steps_total=spent.get(ci, 0) + r["steps_run"])
                 q = queues[_kind(specs[ci])]
                 if q:
                     nxt = q.popleft()
@@ -449,7 +464,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                             continue
                         cand = [(step - placed[ci], ci) for ci in slot_case
                                 if ci is not None and ci not in restarted and ci not in hedge_of
-                                and _kind(specs[ci]) == sub_kind[g]]
+                                and ci not in cold and _kind(specs[ci]) == sub_kind[g]]
                         if not cand:
                             break
                         ci = max(cand)[1]
@@ -556,7 +571,8 @@ class WarmStarter:
         inits, fallback, t_lr, t_inf = self.initial_states(cases, mode, lr_max_steps)
         fb = None if mode == "cold" else [c.hr_init for c in cases]
         reps, t, restarted = campaign([c.hr_spec for c in cases], inits, slots=slots, max_steps=max_steps,
-                                      fallback_inits=fb, trace=trace)
+                                      fallback_inits=fb, trace=trace,
+                                      cold_ids=[i for i, f in enumerate(fallback) if f])
         for i in restarted:
             fallback[i] = True
         torch.cuda.synchronize()

@@ -233,3 +233,51 @@ def test_hedge_goes_to_the_case_longest_in_its_slot(mock):
     later = [x for x in bs.log if x[0] == "replace" and x[3] >= 2000]
     assert later[0][2] == 3                  # the refill
     assert later[1][2] == 2                  # the first hedge
+
+
+def test_steps_total_counts_the_discarded_warm_attempt(mock):
+    # case 2 stalls from its warm start and restarts from its fallback in its
+    # slot: its total is the warm attempt plus the fallback run, its stopping
+    # step only the latter
+    specs, inits = _cases(1, 6, lambda i, k: 1500)
+    inits[2] = "stuck"
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=3, chunk=100, fallback_inits=fb, stall=(1000, 500, 0.97))
+    assert restarted == [2]
+    bs = mock.instances[0]
+    placed = [x[3] for x in bs.log if x[0] == "replace" and x[2] == 2]
+    assert len(placed) == 1                       # in flight from step 0, restarted once
+    warm_steps = placed[0]
+    assert reps[2]["steps_total"] == reps[2]["steps_run"] + warm_steps
+    assert reps[2]["steps_total"] >= specs[2].tag["cold"] + 1000
+    # cases 0 and 1 finish while cases still wait (no hedges yet); every
+    # total covers at least the run that gave the result
+    assert all(reps[i]["steps_total"] == reps[i]["steps_run"] for i in (0, 1))
+    assert all(r["steps_total"] >= r["steps_run"] for r in reps)
+
+
+def test_steps_total_counts_a_losing_hedge(mock):
+    # a slow warm start that converges before its hedge: its total includes
+    # the hedge's steps
+    specs, inits = _cases(1, 4, lambda i, k: 1000 if i != 3 else 1800)
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=2, chunk=100, fallback_inits=fb)
+    assert restarted == [] and reps[3]["stopping_step"] == 1800
+    bs = mock.instances[0]
+    hedge = [x for x in bs.log if x[0] == "replace" and x[2] == 3 and x[3] > 0]
+    assert hedge, "a hedge ran for case 3"
+    assert reps[3]["steps_total"] > reps[3]["steps_run"]
+
+
+def test_cases_already_at_their_fallback_are_neither_hedged_nor_restarted(mock):
+    # case 3 starts from its fallback (a non-physical prediction): a hedge or
+    # a restart would replay the same trajectory
+    specs, inits = _cases(1, 4, lambda i, k: 1000 if i != 3 else 1800)
+    inits[3] = "stuck"
+    fb = ["fallback"] * len(specs)
+    reps, t, restarted = P.campaign(specs, inits, slots=2, chunk=100, fallback_inits=fb, cold_ids=[3],
+                                    stall=(1500, 500, 0.97), max_steps=4000)
+    assert restarted == []
+    bs = mock.instances[0]
+    assert not [x for x in bs.log if x[0] == "replace" and x[2] == 3 and x[3] > 1000]
+    assert not reps[3]["converged"] and reps[3]["steps_total"] == reps[3]["steps_run"]
