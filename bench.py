@@ -194,6 +194,27 @@ def ragged_grid(n, seed=0):
     return out
 
 
+MIXED_GEOMETRIES = ("diamond", "naca0012", "naca2412", "naca4412")
+MIXED_H = (0.1, 0.05, 0.025, 0.0175, 0.0125)
+
+
+def mixed_grid(n, seed=0):
+    """(geometry, h, M, alpha) of the ragged plain-method configuration: n
+    cases over the 20 meshes {diamond, NACA0012, NACA2412, NACA4412} x
+    h in {0.1, 0.05, 0.025, 0.0175, 0.0125}, taken round robin, with
+    (M, alpha) drawn by default_rng(seed) from M = 0.70 + 0.01 i
+    (i = 0..230) x alpha = -5 + 0.5 j (j = 0..20)."""
+    rng = np.random.default_rng(seed)
+    meshes = [(g, h) for h in MIXED_H for g in MIXED_GEOMETRIES]
+    out = []
+    for i in range(n):
+        m = round(0.70 + 0.01 * int(rng.integers(0, 231)), 2)
+        a = -5.0 + 0.5 * int(rng.integers(0, 21))
+        g, h = meshes[i % len(meshes)]
+        out.append((g, h, m, a))
+    return out
+
+
 def make_workload(meshes, grid, cap_ref=0):
     """WarmCase list (input order) with HR reference fields from fixed-time
     solves (one batch for all cases)."""
