@@ -87,6 +87,8 @@ SIGNATURES = {
     "esr_residual": (C.c_int, [C.c_void_p, C.POINTER(Opts), C.c_void_p, C.c_void_p, C.c_void_p]),
     "esr_step": (C.c_int, [C.c_void_p, C.POINTER(Opts), C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
                            C.c_void_p]),
+    "esr_step_noalloc": (C.c_int, [C.c_void_p, C.POINTER(Opts), C.c_int32, C.c_float, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
     "esr_get_dt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_double, C.c_double, f32p, C.c_void_p]),
     "esr_forces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.POINTER(C.c_double),
                              C.POINTER(C.c_double), C.c_void_p]),

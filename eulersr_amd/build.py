@@ -23,7 +23,7 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = os.environ.get("ESR_OFFLOAD_ARCH", "gfx950")
 
 HOST_SRC = ["esr_mesh.cpp", "esr_api.cpp", "esr_mlp_api.cpp"]
-DEV_SRC = ["esr_kernels.hip", "esr_mlp.hip"]
+DEV_SRC = ["esr_kernels.hip", "esr_mlp.hip", "esr_noalloc.hip"]
 HEADERS = ["esr_internal.h", "esr_launch.h", "esr_mlp.h"]
 
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall",

@@ -24,7 +24,9 @@
 
 using namespace esr;
 
-namespace {
+// Helpers shared with the allocation-free entry point (esr_noalloc.hip);
+// their float arithmetic is compiled here, without contraction.
+namespace esr {
 
 Phys make_phys(double gamma, double M) {
   Phys p;
@@ -69,12 +71,6 @@ float sponge_strength(const esr_opts& o, float width) {
   return 1.0f * (U + a) / width;
 }
 
-struct Stage {
-  int mode;
-  int ws;   // 0: W0, 1: W1, 2: W2
-  int out;  // 0: W0, 1: W1, 2: W2
-};
-
 std::vector<Stage> stages_of(int scheme) {
   switch (scheme) {
     case ESR_SCHEME_EE: return {{MODE_EE, 0, 1}};
@@ -85,6 +81,21 @@ std::vector<Stage> stages_of(int scheme) {
     default: throw Error(ESR_ERR_KEY, "unknown time scheme (expected EE, RK2, RK4, SRK2, SSP_RK2)");
   }
 }
+
+// parameters of a case from its options (dt set later)
+CaseDev case_params(const esr_mesh* m, const esr_opts& o) {
+  CaseDev cd{};
+  cd.value = make_float4(o.value[0], o.value[1], o.value[2], o.value[3]);
+  const float width = o.sponge_width > 0 ? o.sponge_width : 0.2f * (m->y_max - m->y_min);
+  cd.strength = o.sponge_strength > 0 ? o.sponge_strength : sponge_strength(o, width);
+  cd.dt = cd.dt_half = cd.dt_sixth = 0.f;
+  cd.active = 1;
+  return cd;
+}
+
+}  // namespace esr
+
+namespace {
 
 struct CaseSpec {
   const esr_mesh* mesh;
@@ -105,7 +116,7 @@ std::map<std::vector<int64_t>, TuneChoice> g_tune;
 
 // stage path of engines built from now on (esr_set_stage_path)
 std::atomic<int32_t> g_stage_path{ESR_STAGE_AUTO};
-int32_t resolved_stage_path() {
+int32_t esr::resolved_stage_path() {
   const int32_t p = g_stage_path.load();
   return p == ESR_STAGE_FACE ? ESR_STAGE_FACE : ESR_STAGE_CELL;
 }
@@ -418,17 +429,7 @@ struct Engine {
                            reinterpret_cast<float4*>(W), st);
   }
 
-  // per-case dt minimum of W0 (times cfl) -> device dt params and dt_d
-  // parameters of a case from its options (dt set later)
-  static CaseDev case_params(const esr_mesh* m, const esr_opts& o) {
-    CaseDev cd{};
-    cd.value = make_float4(o.value[0], o.value[1], o.value[2], o.value[3]);
-    const float width = o.sponge_width > 0 ? o.sponge_width : 0.2f * (m->y_max - m->y_min);
-    cd.strength = o.sponge_strength > 0 ? o.sponge_strength : sponge_strength(o, width);
-    cd.dt = cd.dt_half = cd.dt_sixth = 0.f;
-    cd.active = 1;
-    return cd;
-  }
+  static CaseDev case_params(const esr_mesh* m, const esr_opts& o) { return esr::case_params(m, o); }
 
   // dt of global case gc alone from its current state (the other cases keep theirs)
   float compute_dt_one(int32_t gc, float cfl, hipStream_t st) {

@@ -29,6 +29,21 @@ struct LaunchSet {
 
 constexpr int BLOCK = 256;  // maximum block of the stage kernels; block of the reductions
 
+// Host helpers (esr_api.cpp): physical constants of a group, option checks,
+// the sponge strength, the stage sequence of a time scheme, the per-case
+// parameters of one case, the stage path of engines built from now on.
+Phys make_phys(double gamma, double M);
+void validate_opts(const esr_opts* o);
+float sponge_strength(const esr_opts& o, float width);
+struct Stage {
+  int mode;
+  int ws;   // input state: 0 W0, 1 W1, 2 W2
+  int out;  // output state: 0 W0, 1 W1, 2 W2
+};
+std::vector<Stage> stages_of(int scheme);
+CaseDev case_params(const esr_mesh* m, const esr_opts& o);
+int32_t resolved_stage_path();
+
 void launch_recon(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, float4* rec, hipStream_t st);
 void launch_lf_cmax(const LaunchSet& ls, const CaseDev* cases, const float4* Ws, const float4* rec,
                     hipStream_t st);

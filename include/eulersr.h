@@ -124,6 +124,17 @@ esr_status esr_residual(esr_mesh* mesh, const esr_opts* opts, const float* W, fl
 esr_status esr_step(esr_mesh* mesh, const esr_opts* opts, int32_t scheme, float dt,
                     const float* W, float* W_out, void* stream);
 
+/* esr_step without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * kernels, same results bit for bit; every buffer of the call lives in one
+ * fixed 18,887,424-byte device array per GPU (held until the process ends),
+ * which limits meshes to 131,072 cells and 262,144 logical faces and the
+ * sponge to its default width (ESR_ERR_VALUE beyond). Calls that overlap on
+ * one GPU share that array and race. The mesh's device must be current. */
+esr_status esr_step_noalloc(esr_mesh* mesh, const esr_opts* opts, int32_t scheme, float dt,
+                            const float* W, float* W_out, void* stream);
+
 /* helper.get_dt(W, mesh, CFL, gamma, M) -- helper.py:8-18 (synchronises). */
 esr_status esr_get_dt(esr_mesh* mesh, const float* W, float cfl, double gamma, double M,
                       float* dt_out, void* stream);
