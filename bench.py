@@ -258,6 +258,183 @@ def stage_kernels(cases, reps, launch=None, inits=None):
     return kernel_table(sizes, ms), sizes, tuning
 
 
+def cd_history(args):
+    """How settled CD is where the protocol stops: the first `args.cd_history`
+    cases of the campaign's grid, each solved three ways (freestream, IDW
+    and network warm starts) with the protocol's stop (thresholds from the
+    reference fields) and, separately, without any stop for
+    `args.cd_history_steps` steps, CD read every 100 steps. Reports |dCD|
+    between warm and cold stops, how far each stop is from its own long-run
+    CD (mean over the last 1,000 steps), the long-run |dCD| between starts,
+    and the CD range over the last 5,000 steps (a limit cycle's amplitude)."""
+    import copy as _copy
+    from eulersr_amd import pipeline as P
+    from eulersr_amd.solve import BatchSolver
+    meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
+    train_grid = [(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
+    starter = P.WarmStarter()
+    for g, (lm, hm) in meshes.items():
+        pairs = P.training_pairs(g, lm, hm, train_grid)
+        kmap = P.WS.KnnMap(lm, hm)
+        net, _ = P.train(kmap, pairs, steps=args.train_steps, batch=args.train_batch, seed=0)
+        starter.add(g, lm, hm, net=net, kmap=kmap)
+        del pairs
+    grid = ragged_grid(args.cases)[:args.cd_history]
+    cases, ref, _ = make_workload(meshes, grid)
+    n, S, CH = len(cases), int(args.cd_history_steps), 100
+    runs = {}
+    for mode in ("cold", "idw", "mlp"):
+        inits, _, _, _ = starter.initial_states(cases, mode)
+        for kind in ("stop", "free"):
+            specs = [_copy.copy(c.hr_spec) for c in cases]
+            for sp in specs:
+                sp.max_steps = S
+                if kind == "free":
+                    sp.threshold = 1e-30
+            bs = BatchSolver(specs, scheme="SRK2", threshold=1.0, check_every=P.CHECK_EVERY, patience=P.PATIENCE,
+                             autotune=False, launch=P.LAUNCH or ("cell", 64, True))
+            try:
+                for i, W in enumerate(inits):
+                    bs.set_state(i, W)
+                bs.prepare()
+                cd, step = [], 0
+                while step < S:
+                    step = bs.run(step + CH)
+                    reps = bs.reports()
+                    cd.append([r["cd"] for r in reps])
+                    if kind == "stop" and all(r["converged"] or r["steps_run"] >= r["N"] for r in reps):
+                        break
+                reps = bs.reports()
+            finally:
+                bs.close()
+            runs[(mode, kind)] = dict(cd=np.array(cd), conv=np.array([r["converged"] for r in reps]),
+                                      stop=np.array([r["stopping_step"] for r in reps]))
+        log(f"CD histories: {mode} done")
+    cold_stop = runs[("cold", "stop")]["cd"][-1]
+    longrun = {m: runs[(m, "free")]["cd"][-10:].mean(axis=0) for m in ("cold", "idw", "mlp")}
+    amp = {m: np.ptp(runs[(m, "free")]["cd"][-50:], axis=0) for m in ("cold", "idw", "mlp")}
+
+    def stats(d, mask=None):
+        d = np.asarray(d) if mask is None else np.asarray(d)[mask]
+        return {"median": float(np.median(d)), "within_1e-4": int(np.sum(d <= 1e-4)), "cases": int(d.size)}
+    out = {"metric": "CD settledness at the protocol's stop", "unit": "|dCD|", "cases": n, "free_steps": S,
+           "cold_stop_vs_own_long_run": stats(np.abs(cold_stop - longrun["cold"])),
+           "cold_cd_range_last_5000_steps": stats(amp["cold"]),
+           "cd_range_over_1e-4_cases": int(np.sum(amp["cold"] > 1e-4))}
+    for m in ("idw", "mlp"):
+        ws, conv = runs[(m, "stop")]["cd"][-1], runs[(m, "stop")]["conv"]
+        out[m] = {"converged_without_restart": int(conv.sum()),
+                  "stop_dcd_vs_cold_stop": stats(np.abs(ws - cold_stop), conv),
+                  "stop_vs_own_long_run": stats(np.abs(ws - longrun[m]), conv),
+                  "long_run_dcd_vs_cold_long_run": stats(np.abs(longrun[m] - longrun["cold"])),
+                  "stop_steps_median": float(np.median(runs[(m, "stop")]["stop"][conv]))}
+    out["cold_stop_steps_median"] = float(np.median(runs[("cold", "stop")]["stop"]))
+    return out
+
+
+def calibration_copy():
+    """Counter calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE passes: a
+    plain device copy of known size (1 GiB, larger than every cache), twice;
+    returns the bytes copied each way."""
+    import torch
+    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    n = x.numel() * 4
+    del x, y
+    return n
+
+
+def kernels_only(args):
+    """The stage kernels at a batch of `args.kernels` HR cases (the first
+    cases of the campaign's grid, freestream starts advanced 10 steps), with
+    the launch pinned by --launch or autotuned; bytes of the state and
+    scratch slabs that one stage touches (the working set beside the
+    Infinity Cache's 256 MB)."""
+    from eulersr_amd import pipeline as P
+    meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
+    grid = ragged_grid(args.kernels)
+    specs, inits = [], []
+    for g, m, a, fl in grid:
+        hs, hi = P.make_cases(meshes[g][1], g, [(m, a)], flux=fl)
+        specs.append(hs[0])
+        inits.append(hi[0])
+
+    class C:        # stage_kernels takes WarmCase-like objects
+        def __init__(self, sp, w):
+            self.hr_spec, self.hr_init = sp, w
+    kern, sizes, tuning = stage_kernels([C(s, w) for s, w in zip(specs, inits)], args.kernel_reps, P.LAUNCH)
+    items, cells, f_items, faces = sizes
+    # per cell work-item: W0, W1 (16 B each), three face states (48 B); per
+    # face work-item: one flux (16 B, face path)
+    working = items * (16 + 16 + 48) + f_items * 16
+    for name, k in kern.items():
+        log(f"{name}: {k['us_per_launch']:.1f} us/launch, {k['bytes_per_launch'] / 1e6:.1f} MB -> {k['gbs']:.0f} GB/s")
+    dom = max(PATH_KERNELS[tuning["path"]], key=lambda k: kern[k]["us_per_launch"])
+    calibration_copy()
+    return {"metric": "stage kernel HBM rate", "unit": "GB/s", "cases": args.kernels, "tuning": tuning,
+            "launch_sizes": dict(zip(("cell_items", "group_cells", "face_items", "group_faces"), sizes)),
+            "state_and_scratch_bytes": working, "stage_kernels": kern, "dominant_kernel": dom,
+            "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": kern[dom]["gbs"] / HBM_PEAK_GBS, "kernel": dom}}
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def mlp_kernels(args):
+    """The warm-start network's kernels (BF16 MFMA GEMMs with FP32
+    accumulation, widths 48-256-256-256-256-4) at two shapes: the training
+    mini-batch (16,384 rows: forward, loss gradient, AdamW) and inference
+    over the campaign's HR rows (one geometry's 205 cases x 28k cells, in the
+    network's row chunks). Timed with HIP events on torch's current stream,
+    which the network's calls use."""
+    import torch
+    from eulersr_amd import warmstart as WS
+    net = WS.WarmNet(in_dim=WS.n_features(), max_rows=1 << 15, seed=0)
+    widths = list(net.widths)
+    flops_row = 2 * sum(a * b for a, b in zip(widths[:-1], widths[1:]))
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    out = {"metric": "MLP kernels, BF16 MFMA", "unit": "TFLOP/s", "widths": widths,
+           "flops_per_row_forward": flops_row, "row_chunk": net.max_rows}
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    rows = 16384
+    X = torch.randn(rows, widths[0], device="cuda", generator=gen).to(torch.bfloat16)
+    T = torch.randn(rows, widths[-1], device="cuda", generator=gen)
+    t_f = timed(lambda: net.forward(X), 50)
+    t_g = timed(lambda: net.loss_grad(X, T), 50)
+    t_a = timed(lambda: net.adamw(1e-4), 50)
+    # a training step: forward + backward = 3x the forward's GEMM flops
+    out["training"] = {"rows": rows, "forward_s": t_f, "loss_grad_s": t_g, "adamw_s": t_a,
+                       "forward_tflops": flops_row * rows / t_f / 1e12,
+                       "train_step_tflops": 3 * flops_row * rows / (t_g + t_a) / 1e12}
+    out["training"]["forward_frac_of_mfma_peak"] = out["training"]["forward_tflops"] / MFMA_BF16_PEAK_TFLOPS
+    out["training"]["train_step_frac_of_mfma_peak"] = out["training"]["train_step_tflops"] / MFMA_BF16_PEAK_TFLOPS
+    rows_inf = 205 * 28120
+    Xi = torch.randn(rows_inf, widths[0], device="cuda", generator=gen).to(torch.bfloat16)
+    t_i = timed(lambda: net.forward(Xi), 5)
+    out["inference"] = {"rows": rows_inf, "forward_s": t_i, "tflops": flops_row * rows_inf / t_i / 1e12,
+                        "frac_of_mfma_peak": flops_row * rows_inf / t_i / 1e12 / MFMA_BF16_PEAK_TFLOPS}
+    log(f"MLP: training forward {t_f * 1e6:.1f} us ({out['training']['forward_tflops']:.1f} TFLOP/s), "
+        f"loss+grad {t_g * 1e6:.1f} us, adamw {t_a * 1e6:.1f} us; inference {rows_inf} rows {t_i * 1e3:.2f} ms "
+        f"({out['inference']['tflops']:.1f} TFLOP/s)")
+    net.close()
+    return out
+
+
 def plain_single(cases, max_steps):
     """The plain single-case GPU solver: each case alone, cold start."""
     from eulersr_amd import pipeline as P
@@ -511,12 +688,23 @@ def main():
                          "the legs the JSON line needs always run; the IDW campaign, the 6-case configuration, "
                          "the CD-drift check and further timed campaigns (up to --steps) only while it lasts")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="seconds of the timed CPU-baseline sample")
+    ap.add_argument("--kernels", type=int, default=0,
+                    help="instead of the headline: time the stage kernels alone on this many HR cases of the "
+                         "campaign's grid (freestream starts advanced 10 steps; one JSON line)")
+    ap.add_argument("--cd-history", type=int, default=0,
+                    help="instead of the headline: CD settledness at the protocol's stop on this many cases of "
+                         "the campaign's grid (one JSON line)")
+    ap.add_argument("--cd-history-steps", type=int, default=40000)
+    ap.add_argument("--mlp", action="store_true",
+                    help="instead of the headline: time the warm-start network's BF16 MFMA kernels at the "
+                         "training shape and at the campaign's inference shape (one JSON line)")
     args = ap.parse_args()
     if args.profile:
         args.max_steps = min(args.max_steps, 2000)
         args.train_steps = min(args.train_steps, 200)
         args.no_cpu_baseline = True
         args.no_small = True
+        args.steps = 1
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: the cases of a batch are independent and would be sharded "
@@ -537,6 +725,15 @@ def main():
         P.AUTOTUNE, P.LAUNCH = False, (p_, int(b_), bool(int(x_)))
     if args.learned_config:
         print(json.dumps(learned_config(args)), flush=True)
+        return 0
+    if args.kernels:
+        print(json.dumps(kernels_only(args)), flush=True)
+        return 0
+    if args.mlp:
+        print(json.dumps(mlp_kernels(args)), flush=True)
+        return 0
+    if args.cd_history:
+        print(json.dumps(cd_history(args)), flush=True)
         return 0
     log(f"building meshes h={args.h}, LR h={args.h_lr}")
     meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
@@ -764,15 +961,7 @@ def main():
         "wall_s": time.time() - T_START,
     }
     if args.profile:
-        # counter calibration: a plain device copy of known size (1 GiB,
-        # larger than every cache), copied twice
-        x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-        y = torch.empty_like(x)
-        for _ in range(2):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        out["calibration_copy_bytes_each_way"] = x.numel() * 4
-        del x, y
+        out["calibration_copy_bytes_each_way"] = calibration_copy()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output for the solver kernels.
 
-    python -m eulersr_amd.profiling trace <dir>/<name>_kernel_trace.csv
+    python -m eulersr_amd.profiling trace <dir>/<name>_kernel_trace.csv [--by-grid]
     python -m eulersr_amd.profiling pmc <fetch_counter_collection.csv> <write_counter_collection.csv>
     python -m eulersr_amd.profiling valu <counter_collection.csv>
 
@@ -29,17 +29,25 @@ def short(name: str) -> str:
     return (m.group(1) + (m.group(2) or "")) if m else name[:40]
 
 
-def trace_summary(path, tail=50):
+def trace_summary(path, tail=50, by_grid=False):
+    """by_grid: one line per (kernel, grid size), which separates the
+    shapes of one kernel (the GEMM's layers and passes)."""
     rows = list(csv.DictReader(open(path)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+
+    def key(r):
+        k = short(r["Kernel_Name"])
+        if by_grid:
+            k += "@" + "x".join(r.get(f"Grid_Size_{a}", "") for a in "XYZ")
+        return k
     by = defaultdict(list)
     for r in rows:
-        by[short(r["Kernel_Name"])].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        by[key(r)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     out = []
     out.append(f"{'kernel':34s} {'calls':>7s} {'avg_us':>9s} {'tail_avg_us':>12s} {'grid':>10s}")
     grids = {}
     for r in rows:
-        grids[short(r["Kernel_Name"])] = r.get("Grid_Size_X", "")
+        grids[key(r)] = r.get("Grid_Size_X", "")
     for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
         t = v[-tail:] if len(v) >= tail else v
         out.append(f"{k:34s} {len(v):7d} {statistics.mean(v):9.2f} {statistics.mean(t):12.2f} {grids[k]:>10s}")
@@ -144,6 +152,7 @@ def main():
     t = sub.add_parser("trace")
     t.add_argument("csv")
     t.add_argument("--tail", type=int, default=50)
+    t.add_argument("--by-grid", action="store_true")
     p = sub.add_parser("pmc")
     p.add_argument("fetch_csv")
     p.add_argument("write_csv")
@@ -157,7 +166,7 @@ def main():
     v.add_argument("--tail", type=int, default=50)
     a = ap.parse_args()
     if a.cmd == "trace":
-        print(trace_summary(a.csv, a.tail))
+        print(trace_summary(a.csv, a.tail, a.by_grid))
     elif a.cmd == "valu":
         print(valu_summary(a.csv, a.tail))
     else:
