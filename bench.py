@@ -624,20 +624,45 @@ def learned_config(args):
         ref, _ = P.fixed_time(hs, hi)
         cases += P.warm_cases(g, lm, hm, test, ref)
     starter.campaign(cases[:4], mode="mlp", slots=4, max_steps=20, lr_max_steps=20)
-    runs = {}
+    runs, full = {}, {}
     for mode in ("mlp", "idw", "cold"):
         r = starter.campaign(cases, mode=mode, slots=args.slots, max_steps=args.max_steps)
+        full[mode] = r
         st = [x["steps_total"] for x in r["reports"] if x["converged"]]
         runs[mode] = dict(total_s=r["total_s"], hr_s=r["hr_s"], lr_s=r["lr_s"], inference_s=r["infer_s"],
                           converged=int(sum(x["converged"] for x in r["reports"])),
                           fallback=int(sum(r["fallback"])), steps_median=float(np.median(st)) if st else None,
-                          steps_mean=float(np.mean(st)) if st else None, cases_per_hour=rate(r),
-                          steps=[x["steps_total"] for x in r["reports"]],
-                          cd=[x["cd"] for x in r["reports"]])
+                          steps_mean=float(np.mean(st)) if st else None, steps_sum=int(sum(st)),
+                          cases_per_hour=rate(r))
         log(f"{mode}: {r['total_s']:.2f}s (HR {r['hr_s']:.2f}s), converged {runs[mode]['converged']}/{len(cases)}, "
-            f"steps median {runs[mode]['steps_median']} mean {runs[mode]['steps_mean']}, "
+            f"steps (every run counted) median {runs[mode]['steps_median']} mean {runs[mode]['steps_mean']}, "
             f"fallback {runs[mode]['fallback']}")
-    dcd = [abs(a - b) for a, b in zip(runs["mlp"]["cd"], runs["cold"]["cd"])]
+    cold = full["cold"]["reports"]
+    out_modes = {}
+    for mode in ("mlp", "idw"):
+        reps = full[mode]["reports"]
+        dcd = [abs(a["cd"] - b["cd"]) for a, b in zip(reps, cold)]
+        # warm attempts given up (stall, divergence, a hedge that converged
+        # first): how far their last in-loop CD was from the plain solve's,
+        # and how far their last check residual was above the threshold
+        gave_up = [(abs(d["cd"] - cold[i]["cd"]), d["last_rel"] / c.hr_spec.threshold)
+                   for i, (x, c) in enumerate(zip(reps, cases)) for d in x.get("discarded", [])]
+        out_modes[mode] = {
+            "abs_dcd_vs_cold": {"median": float(np.median(dcd)), "within_1e-4": int(sum(d <= 1e-4 for d in dcd))},
+            "qualified_cases_per_hour": sum(x["converged"] and d <= DCD_TOL for x, d in zip(reps, dcd))
+            / full[mode]["total_s"] * 3600.0,
+            "speedup_iterations_of_sums": runs["cold"]["steps_sum"] / max(1, runs[mode]["steps_sum"]),
+            "speedup_iterations_median": float(np.median([c["steps_total"] / max(1, x["steps_total"])
+                                                          for x, c in zip(reps, cold)
+                                                          if x["converged"] and c["converged"]])),
+            "speedup_hr_time": full["cold"]["hr_s"] / full[mode]["hr_s"],
+            "speedup_pipeline": full["cold"]["hr_s"] / full[mode]["total_s"],
+            "given_up_attempts": len(gave_up),
+            "given_up_abs_dcd_vs_cold_median": float(np.median([g[0] for g in gave_up])) if gave_up else None,
+            "given_up_residual_over_threshold_median": float(np.median([g[1] for g in gave_up])) if gave_up else None,
+            "given_up_within_1e-4": int(sum(g[0] <= 1e-4 for g in gave_up)),
+        }
+        log(f"{mode} vs cold: {out_modes[mode]}")
     return {
         "metric": "HR steps and wall time to the reference-field threshold, learned warm start vs IDW and "
                   "freestream starts",
@@ -648,11 +673,10 @@ def learned_config(args):
                                f"{n_test} held-out cases per geometry solved with {args.slots} slots",
                    "pairs": args.learned_pairs, "test_cases": len(cases)},
         "training": training,
-        "runs": {m: {k: v for k, v in r.items() if k not in ("steps", "cd")} for m, r in runs.items()},
-        "steps_ratio_cold_over_mlp": (runs["cold"]["steps_mean"] / runs["mlp"]["steps_mean"]
-                                      if runs["mlp"]["steps_mean"] else None),
-        "hr_time_ratio_cold_over_mlp": runs["cold"]["hr_s"] / runs["mlp"]["hr_s"],
-        "abs_dcd_vs_cold": {"median": float(np.median(dcd)), "within_1e-4": int(sum(d <= 1e-4 for d in dcd))},
+        "runs": runs,
+        "vs_cold": out_modes,
+        "hr_time_ratio_cold_over_mlp": full["cold"]["hr_s"] / full["mlp"]["hr_s"],
+        "hr_time_ratio_idw_over_mlp": full["idw"]["hr_s"] / full["mlp"]["hr_s"],
     }
 
 
@@ -801,7 +825,7 @@ def main():
     if not args.profile:
         cold = starter.campaign(cases, mode="cold", slots=args.slots, max_steps=args.max_steps)
         log(f"cold-start campaign: {cold['total_s']:.2f}s -> {rate(cold):.0f} cases/hour")
-    dcd_all = qual = None
+    dcd_all = qual = gave_up = None
     if cold is not None:
         dcd_all = [abs(w["cd"] - c["cd"]) if (w["converged"] and c["converged"]) else float("inf")
                    for w, c in zip(warm["reports"], cold["reports"])]
@@ -809,6 +833,11 @@ def main():
         fin = [d for d in dcd_all if d != float("inf")]
         log(f"|dCD| warm vs plain over all {len(cases)} cases: median {np.median(fin):.1e}, within {DCD_TOL:g}: "
             f"{sum(qual)}/{len(cases)}")
+        # warm attempts given up (stall, divergence, or a hedge that converged
+        # first): their last in-loop CD against the plain solve's, and their
+        # last check residual against the case's threshold
+        gave_up = [(abs(d["cd"] - cold["reports"][i]["cd"]), d["last_rel"] / cases[i].hr_spec.threshold)
+                   for i, x in enumerate(warm["reports"]) for d in x.get("discarded", [])]
 
     # plain single-case GPU solver on a sample of the batch (cold, one at a time)
     sample = list(range(min(args.single_sample, len(cases))))
@@ -938,6 +967,10 @@ def main():
             "median": float(np.median([d for d in dcd_all if d != float("inf")])),
             "p90": float(np.percentile([d for d in dcd_all if d != float("inf")], 90)),
             "within_1e-4": int(sum(d <= DCD_TOL for d in dcd_all)), "cases": len(dcd_all)},
+        "given_up_warm_attempts": None if not gave_up else {
+            "count": len(gave_up), "abs_dcd_vs_plain_median": float(np.median([g[0] for g in gave_up])),
+            "within_1e-4": int(sum(g[0] <= DCD_TOL for g in gave_up)),
+            "residual_over_threshold_median": float(np.median([g[1] for g in gave_up]))},
         "steps_warm": steps_stats(warm), "steps_cold": steps_stats(cold) if cold else None,
         "steps_idw": steps_stats(idw) if idw else None,
         "slots": args.slots,

@@ -282,7 +282,10 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     Each report carries ``steps_total``: the steps the case took in every
     slot it ran in -- a warm attempt that was restarted or lost to its hedge,
     a hedge that lost, and the run that gave the result -- so step counts and
-    cell-updates of warm starts include the work they discarded.
+    cell-updates of warm starts include the work they discarded; and
+    ``discarded``: the last in-loop CD, CL, check residual and steps of each
+    warm attempt it gave up (stall, divergence, or a hedge that converged
+    first).
 
     Returns (reports in input order, seconds of solving, restarted case ids)."""
     import math
@@ -350,6 +353,11 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
         restarted = []
         cold = set(cold_ids)
         spent = {}                             # case id -> steps of its discarded runs
+        dropped = {}                           # case id -> last reports of its discarded warm attempts
+
+        def discard(ci, r):
+            dropped.setdefault(ci, []).append(dict(cd=r["cd"], cl=r["cl"], last_rel=r["last_rel"],
+                                                   steps_run=r["steps_run"], diverged=r["diverged"]))
         changed = False
         slot_hedge = [None] * len(slot_case)   # case id whose fallback runs in the slot
         hedge_of = {}                          # case id -> slot of its hedge
@@ -379,7 +387,8 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     retire(p, ci)
                     slot_case[p] = None
                     spent[ci] = spent.get(ci, 0) + reps[p]["steps_run"]
-                    results[ci] = dict(r, steps_total=spent[ci] + r["steps_run"])
+                    discard(ci, reps[p])
+                    results[ci] = dict(r, steps_total=spent[ci] + r["steps_run"], discarded=dropped.get(ci, []))
                     restarted.append(ci)
                     slot_hedge[j] = None
                     del hedge_of[ci]
@@ -401,6 +410,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                         hist[ci].append((r["steps_run"], r["last_rel"]))
                     if r["diverged"] or (not done and stalled(hist[ci], r["steps_run"], *stall)):
                         spent[ci] = spent.get(ci, 0) + r["steps_run"]
+                        discard(ci, r)
                         if ci in hedge_of:
                             # its fallback already runs: that slot carries the case on
                             h = hedge_of.pop(ci)
@@ -420,7 +430,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     retire(h, ci)
                     slot_hedge[h] = None
                     spent[ci] = spent.get(ci, 0) + reps[h]["steps_run"]
-                results[ci] = dict(r, steps_total=spent.get(ci, 0) + r["steps_run"])
+                results[ci] = dict(r, steps_total=spent.get(ci, 0) + r["steps_run"], discarded=dropped.get(ci, []))
                 q = queues[_kind(specs[ci])]
                 if q:
                     nxt = q.popleft()

@@ -61,7 +61,7 @@ class MockSolver:
         hist = 1e-3 if self.init[j] == "stuck" else 1e-3 / (1 + ran)
         return dict(converged=bool(ran >= self._cost(j)), diverged=False, steps_run=min(ran, N), N=N,
                     stopping_step=self._cost(j) if conv else N, last_rel=hist, kind=P._kind(sp), group=sp.group,
-                    case=sp.tag.get("id"))
+                    case=sp.tag.get("id"), cd=float(ran), cl=0.0)
 
     def run(self, until):
         assert until > self.step
@@ -250,6 +250,9 @@ def test_steps_total_counts_the_discarded_warm_attempt(mock):
     warm_steps = placed[0]
     assert reps[2]["steps_total"] == reps[2]["steps_run"] + warm_steps
     assert reps[2]["steps_total"] >= specs[2].tag["cold"] + 1000
+    # the abandoned warm attempt is on record, with its last in-loop CD
+    assert len(reps[2]["discarded"]) == 1 and reps[2]["discarded"][0]["steps_run"] == warm_steps
+    assert all(r["discarded"] == [] for i, r in enumerate(reps) if i != 2)
     # cases 0 and 1 finish while cases still wait (no hedges yet); every
     # total covers at least the run that gave the result
     assert all(reps[i]["steps_total"] == reps[i]["steps_run"] for i in (0, 1))

@@ -226,6 +226,7 @@ def test_solve_loop_matches_oracle():
     assert rep["stopping_step"] == info["stopping_step"]
     assert abs(rep["final_residual"] - info["final_residual"]) <= 1e-3 * info["final_residual"]
     d = np.abs(Wg.astype(np.float64) - Wr) / np.maximum(np.abs(Wr), 1.0)
+    print(f"state after the loop ({rep['steps_run']} steps): max relative difference {d.max():.2e}", flush=True)
     assert d.max() < 1e-4
 
 
