@@ -173,6 +173,7 @@ struct Item {
   int32_t idx;  // cell (cell kernels) or logical face (face kernels)
   int32_t c;
   uint32_t e;   // element offset in the group's state slab (cell kernels)
+  bool active;  // the case still runs (LAZY mapping: stores only when true)
 };
 
 // XCD-aware block order (a tuning choice): the hardware deals workgroup p to
@@ -188,7 +189,10 @@ __device__ __forceinline__ int32_t logical_block(int xcd) {
   return (int32_t)(blockIdx.x % N_XCD) * per + (int32_t)(blockIdx.x / N_XCD);
 }
 
-template <bool FACES>
+// LAZY: a work-item of a stopped case is mapped too, with it.active = false,
+// so the kernel issues its loads without first waiting for the case's flag
+// and only skips its stores; otherwise such an item returns false at once.
+template <bool FACES, bool LAZY = false>
 __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, const BlockRef* __restrict__ blk_group,
                                          const CaseDev* __restrict__ cases, int xcd, Item& it) {
   const BlockRef e = blk_group[logical_block(xcd)];
@@ -205,7 +209,12 @@ __device__ __forceinline__ bool map_item(const GroupDev* __restrict__ groups, co
   const int32_t idx = (int32_t)q;
   const int32_t c = (int32_t)(t - q * (uint32_t)C);
   if (idx >= (FACES ? G->m.nf : G->m.n)) return false;
-  if (!cases[G->case0 + c].active) return false;
+  if (LAZY) {
+    it.active = cases[G->case0 + c].active != 0;
+  } else {
+    if (!cases[G->case0 + c].active) return false;
+    it.active = true;
+  }
   it.G = G;
   it.idx = idx;
   it.c = c;
@@ -260,7 +269,7 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
                                                  const CaseDev* __restrict__ cases,
                                                  const float4* __restrict__ Ws, float4* __restrict__ rec, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
+  if (!map_item<false, true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
@@ -326,6 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
     phi[v] = fminf(fminf(p3[0], p3[1]), p3[2]);
   }
 
+  if (!it.active) return;
   // limited face states (primitives); no slope on marker > 1 faces
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -782,7 +792,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
                                                 const CaseDev* __restrict__ cases, const float4* __restrict__ Ws,
                                                 const float4* __restrict__ rec, float4* __restrict__ fx, int xcd) {
   Item it;
-  if (!map_item<true>(groups, blk_group, cases, xcd, it)) return;
+  if (!map_item<true, true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const CaseDev& cs = cases[G->case0 + it.c];
@@ -794,6 +804,7 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   float F[4];
   class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, G->ph, lf_c, F);
+  if (!it.active) return;
   st(fx + G->fx_off, fel(G, it.idx, it.c), make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]));
 }
 
@@ -892,7 +903,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
                                                   const float4* __restrict__ Ws, const float4* W0,
                                                   const float4* __restrict__ fx, float4* out, float4* acc, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
+  if (!map_item<false, true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const int32_t n = m.n, cell = it.idx;
@@ -905,6 +916,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
     const float4 f = ld(fg, fel(G, (int32_t)(cf & ~CFACE_NEG), it.c));
     Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
+  if (!it.active) return;
   const int64_t so = G->st_off;
   stage_epilogue<LOCAL>(MODE, G, cs, cell, it.e, ld(Ws + so, it.e), Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
@@ -923,7 +935,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
                                                 const float4* W0, const float4* __restrict__ rec, float4* out,
                                                 float4* acc, int mode, int xcd) {
   Item it;
-  if (!map_item<false>(groups, blk_group, cases, xcd, it)) return;
+  if (!map_item<false, true>(groups, blk_group, cases, xcd, it)) return;
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const Phys& ph = G->ph;
@@ -968,6 +980,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
     class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, ph, lf_c, F);
     Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
   }
+  if (!it.active) return;
   const int64_t so = G->st_off;
   stage_epilogue<false>(mode, G, cs, cell, it.e, Wi, Fk, W0 + so, out + so, acc ? acc + so : acc);
 }
