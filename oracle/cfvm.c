@@ -81,11 +81,22 @@ static consts make_consts(const cf_opts* o) {
   return c;
 }
 
+/* Divisions of the step: exact IEEE quotients by default (numpy's); built
+ * with -DCF_RCP_DIV, a / b becomes a * (1 / b), the reciprocal rounded first
+ * -- the rounding of the GPU kernels, which multiply by reciprocals (of the
+ * hardware rcp, of host-precomputed constants) -- so the two builds measure
+ * what that rounding alone moves after many steps. */
+#ifdef CF_RCP_DIV
+#define FDIV(a, b) ((a) * (1.0f / (b)))
+#else
+#define FDIV(a, b) ((a) / (b))
+#endif
+
 /* getPrimitive (helper.py:37-46): P = (E - 0.5 rho (u^2+v^2)) * M^2 * (gamma-1) */
 static inline void prim_of(const float* W, float m2, float gm1, float* p) {
   const float rho = W[0];
-  const float u = W[1] / rho;
-  const float v = W[2] / rho;
+  const float u = FDIV(W[1], rho);
+  const float v = FDIV(W[2], rho);
   const float P = ((W[3] - 0.5f * rho * (u * u + v * v)) * m2) * gm1;
   p[0] = rho; p[1] = u; p[2] = v; p[3] = P;
 }
@@ -96,7 +107,7 @@ static inline void cons_of(const float* p, float gm1M2, float* W) {
   W[0] = rho;
   W[1] = rho * u;
   W[2] = rho * v;
-  W[3] = P / gm1M2 + 0.5f * rho * (u * u + v * v);
+  W[3] = FDIV(P, gm1M2) + 0.5f * rho * (u * u + v * v);
 }
 
 static inline int64_t wrap(int64_t j, int64_t n) { return j < 0 ? j + n : j; }
@@ -142,7 +153,7 @@ static inline void midpoint(const cf_mesh* m, int64_t f, float* x) {
 
 /* venkatakrishnan with K = 0 (euler_solver.py:36-39) */
 static inline float venkat(float a, float b) {
-  return (a * a + 2.0f * a * b + 0.0f) / (a * a + 2.0f * (b * b) + a * b + 0.0f + 1e-7f);
+  return FDIV(a * a + 2.0f * a * b + 0.0f, a * a + 2.0f * (b * b) + a * b + 0.0f + 1e-7f);
 }
 
 /* The cell's neighbour state after the boundary conditions (conserved). */
@@ -240,32 +251,32 @@ static inline void hllc(const float* WL, const float* WR, float nx, float ny, co
   const float tx = -ny, ty = nx;
   const float utL = uL * tx + vL * ty;
   const float utR = uR * tx + vR * ty;
-  const float aL = sqrtf(fabsf(c->g * PL / rL)) / c->M;
-  const float aR = sqrtf(fabsf(c->g * PR / rR)) / c->M;
+  const float aL = FDIV(sqrtf(fabsf(FDIV(c->g * PL, rL))), c->M);
+  const float aR = FDIV(sqrtf(fabsf(FDIV(c->g * PR, rR))), c->M);
   float SL = fminf(unL - aL, unR - aR);
   float SR = fmaxf(unL + aL, unR + aR);
   if (SL == SR) SL = SL - 1e-6f;
   if (SL == SR) SR = SR + 1e-6f;
   const float EL = WL[3], ER = WR[3];
-  const float FL[4] = {rL * unL, rL * uL * unL + PL * nx / c->M2, rL * vL * unL + PL * ny / c->M2,
-                       (EL + PL / c->M2) * unL};
-  const float FR[4] = {rR * unR, rR * uR * unR + PR * nx / c->M2, rR * vR * unR + PR * ny / c->M2,
-                       (ER + PR / c->M2) * unR};
+  const float FL[4] = {rL * unL, rL * uL * unL + FDIV(PL * nx, c->M2), rL * vL * unL + FDIV(PL * ny, c->M2),
+                       (EL + FDIV(PL, c->M2)) * unL};
+  const float FR[4] = {rR * unR, rR * uR * unR + FDIV(PR * nx, c->M2), rR * vR * unR + FDIV(PR * ny, c->M2),
+                       (ER + FDIV(PR, c->M2)) * unR};
   float den = rL * (SL - unL) - rR * (SR - unR);
   if (den == 0.0f) den = 1e-12f;
-  const float Ss = (PR - PL + rL * unL * (SL - unL) - rR * unR * (SR - unR)) / den;
+  const float Ss = FDIV(PR - PL + rL * unL * (SL - unL) - rR * unR * (SR - unR), den);
   const float PsL = PL + rL * (SL - unL) * (Ss - unL);
   const float PsR = PR + rR * (SR - unR) * (Ss - unR);
   const float Ps = 0.5f * (PsL + PsR);
-  const float rsL = rL * (SL - unL) / (SL - Ss);
+  const float rsL = FDIV(rL * (SL - unL), SL - Ss);
   const float usxL = Ss * nx + utL * tx;
   const float usyL = Ss * ny + utL * ty;
-  const float EsL = ((SL - unL) * EL - PL * unL + Ps * Ss) / (SL - Ss);
+  const float EsL = FDIV((SL - unL) * EL - PL * unL + Ps * Ss, SL - Ss);
   const float UsL[4] = {rsL, rsL * usxL, rsL * usyL, EsL};
-  const float rsR = rR * (SR - unR) / (SR - Ss);
+  const float rsR = FDIV(rR * (SR - unR), SR - Ss);
   const float usxR = Ss * nx + utR * tx;
   const float usyR = Ss * ny + utR * ty;
-  const float EsR = ((SR - unR) * ER - PR * unR + Ps * Ss) / (SR - Ss);
+  const float EsR = FDIV((SR - unR) * ER - PR * unR + Ps * Ss, SR - Ss);
   const float UsR[4] = {rsR, rsR * usxR, rsR * usyR, EsR};
   for (int v = 0; v < 4; ++v) {
     float F;
@@ -373,7 +384,7 @@ static void cell_residual(const cf_mesh* m, const cf_opts* o, const consts* c, c
   for (int v = 0; v < 4; ++v) {
     const float F = (Fs[0][v] + Fs[1][v]) + Fs[2][v];
     const float src = m->bump ? 0.0f : sig * (o->value[v] - Wi[v]);
-    R[v] = F / m->area[i] - src;
+    R[v] = FDIV(F, m->area[i]) - src;
   }
 }
 

@@ -36,31 +36,36 @@ class _Opts(C.Structure):
                 ("sponge_strength", C.c_double)]
 
 
-_lib = None
+LIB_RCP = os.path.join(HERE, "libcfvm_rcp.so")
+_libs = {}
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
-            raise RuntimeError(f"{LIB} is not built (make -C oracle)")
-        L = C.CDLL(LIB)
+def lib(rcp=False):
+    """The restatement (rcp=False), or its build whose divisions go through
+    rounded reciprocals, a * (1 / b) (rcp=True)."""
+    path = LIB_RCP if rcp else LIB
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (make -C oracle)")
+        L = C.CDLL(path)
         L.cf_residual.argtypes = [C.c_void_p] * 6
         L.cf_srk2.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_int64] + [C.c_void_p] * 4
         L.cf_get_dt.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double]
         L.cf_get_dt.restype = C.c_float
         L.cf_set_threads.argtypes = [C.c_int]
         L.cf_max_threads.restype = C.c_int
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def set_threads(n: int):
     lib().cf_set_threads(int(n))
+    if os.path.exists(LIB_RCP):
+        lib(True).cf_set_threads(int(n))
 
 
 def max_threads() -> int:
@@ -128,14 +133,15 @@ def residual(W, cm, **kw):
     return R
 
 
-def srk2_steps(W, cm, dt, n_steps, **kw):
-    """n_steps SSP-RK2 steps (euler_solver.py:635-643); returns the new state."""
+def srk2_steps(W, cm, dt, n_steps, rcp=False, **kw):
+    """n_steps SSP-RK2 steps (euler_solver.py:635-643); returns the new state.
+    rcp: on the build whose divisions go through rounded reciprocals."""
     cm = _cm(cm, kw)
     W = np.array(W, dtype=np.float32, order="C", copy=True)
     W1 = np.empty_like(W)
     k = np.empty_like(W)
     o = _opts(kw)
-    lib().cf_srk2(C.byref(cm.s), C.byref(o), _ptr(W), np.float32(dt), int(n_steps), _ptr(W1), _ptr(k),
+    lib(rcp).cf_srk2(C.byref(cm.s), C.byref(o), _ptr(W), np.float32(dt), int(n_steps), _ptr(W1), _ptr(k),
                   _ptr(cm.grad), _ptr(cm.phi))
     return W
 

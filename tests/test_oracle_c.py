@@ -122,3 +122,31 @@ def test_refuses_what_it_does_not_restate(CF):
         CF.residual(W, cm, flux="HLLC", entropy=True, value=inlet)
     with pytest.raises(TypeError):
         CF.CMesh(F.omesh(mesh, np.float64))
+
+
+@pytest.mark.parametrize("h", [0.025, 0.0125])
+def test_reciprocal_division_alone_within_1e5(CF, h):
+    """How much of the 1e-5 per-cell bar on 100 steps float32 rounding of the
+    divisions alone takes: the restatement against its own build whose
+    divisions are a * (1 / b) with a correctly rounded reciprocal (the GPU's
+    quotients go through a 1-ulp hardware reciprocal, so they differ from
+    correctly rounded ones at least this much). Diamond, M = 2, alpha = 2 deg,
+    HLLC + MUSCL, SSP-RK2 at CFL 0.6 from freestream."""
+    from eulersr_amd import main as M
+    from oracle import fvm as F
+    if not os.path.exists(CF.LIB_RCP):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    mesh = mesh_of("diamond", h)
+    cfg = base_cfg(Mach=2.0, aoa=2.0, case="diamond", flux="HLLC", reconstruction="MUSCL", CFL=0.6)
+    W, inlet = M.initialize(mesh, cfg)
+    sp = M.case_spec(mesh, inlet, cfg)
+    cm = CF.CMesh(F.omesh(mesh, np.float32))
+    dt = CF.get_dt(W, cm, CFL=sp.cfl)
+    kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=sp.kw["value"], entropy=False)
+    pa = F.getPrimitive(CF.srk2_steps(W, cm, dt, 100, **kw).astype(np.float64))
+    pb = F.getPrimitive(CF.srk2_steps(W, cm, dt, 100, rcp=True, **kw).astype(np.float64))
+    U = np.hypot(sp.kw["value"][1], sp.kw["value"][2]) / sp.kw["value"][0]
+    rel = np.abs(pb - pa) / np.maximum(np.abs(pa), np.array([1.0, U, U, 1.0]))
+    print(f"{mesh.n_cells} cells: max rel per primitive {np.array2string(rel.max(axis=0), precision=2)}")
+    # the two builds do differ (the switch reaches the step), by less than the bar
+    assert 0.0 < rel.max() < 1e-5
