@@ -45,6 +45,39 @@ def test_forward_matches_oracle(rows, widths):
     net.close()
 
 
+@pytest.mark.parametrize("rows,widths", [(333, [40, 256, 4]), (4099, [48, 256, 256, 256, 256, 4])])
+def test_forward_is_its_layers_composed_exactly(rows, widths):
+    """The multi-layer forward equals, bit for bit, a chain of one-layer
+    networks with the same weights, ReLU and round-to-nearest-even bfloat16
+    applied on the host between them; and every layer of the chain, fed the
+    GPU's own activations, matches the float64 oracle within 1e-5. The
+    tolerance above for whole networks covers only where the GPU's and the
+    oracle's hidden activations round to neighbouring bfloat16 values."""
+    torch = _dev()
+    from oracle import mlp as OM
+    from eulersr_amd.warmstart import WarmNet
+    print(f"forward composition rows={rows} widths={widths}", flush=True)
+    rng = np.random.default_rng(11 + rows)
+    net = WarmNet(in_dim=widths[0], hidden=widths[1:-1], out_dim=widths[-1], max_rows=2048)
+    p = (rng.normal(0, 1, net.n_params) / np.sqrt(widths[0])).astype(np.float32)
+    net.set_params(p)
+    X = OM.bf16(rng.normal(0, 1, (rows, widths[0])).astype(np.float32))
+    Y = net.forward(_bf16_tensor(torch, X)).cpu().numpy()
+    net.close()
+    H = X
+    for li, (W, b) in enumerate(OM.unpack(p, widths)):
+        one = WarmNet(in_dim=widths[li], hidden=[], out_dim=widths[li + 1], max_rows=2048)
+        one.set_params(np.concatenate([W.reshape(-1), b]))
+        Z = one.forward(_bf16_tensor(torch, H)).cpu().numpy()
+        one.close()
+        ref = H.astype(np.float64) @ OM.bf16(W).astype(np.float64).T + b.astype(np.float64)
+        err = np.abs(Z - ref).max() / (np.abs(ref).max() + 1e-30)
+        print(f"layer {li}: max relative error against float64 {err:.2e}", flush=True)
+        assert err <= 1e-5
+        H = Z if li == len(widths) - 2 else OM.bf16(np.maximum(Z, 0.0))
+    assert np.array_equal(Y.view(np.uint32), H.view(np.uint32))
+
+
 @pytest.mark.parametrize("rows,widths", [(300, [48, 70, 4]), (3000, [48, 70, 4]), (5000, [48, 256, 256, 4])])
 def test_loss_grad_matches_oracle(rows, widths):
     torch = _dev()
