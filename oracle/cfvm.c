@@ -1,5 +1,6 @@
 /* CPU oracle, C restatement (TEST INFRASTRUCTURE ONLY): the reference's
- * explicit finite-volume residual for HLLC + constant/MUSCL reconstruction,
+ * explicit finite-volume residual for HLLC or Roe + constant/MUSCL
+ * reconstruction,
  * the sponge, get_dt and the SSP-RK2 step, in float32, parallel over cells
  * with OpenMP.
  *
@@ -15,6 +16,9 @@
  *   MUSCL                       euler_solver.py:69-98
  *   build_reconstruction        euler_solver.py:106-122
  *   HLLC                        euler_solver.py:273-374
+ *   Roe (R^T, S scaling)        euler_solver.py:145-199, Roe average
+ *                               helper.py:101-127, entropy variables
+ *                               helper.py:48-70
  *   getFlux, face lengths       euler_solver.py:141-143, 376-420
  *   sponge                      helper.py:262-295
  *   residual                    euler_solver.py:602-628
@@ -23,8 +27,13 @@
  * Built with -ffp-contract=off and no fast-math, so every float operation
  * rounds once to float32 in the same order as numpy's elementwise kernels:
  * the results are bit-identical to oracle/fvm.py in float32 (checked by
- * tests/test_oracle_c.py). Python-float constants of the numpy code are
- * formed in double and rounded to float32 where numpy's weak promotion does.
+ * tests/test_oracle_c.py) wherever only IEEE operations are involved (HLLC:
+ * + - * / and sqrt). The Roe dissipation's entropy variables take logf and
+ * powf from the C library, where numpy may use its own vectorised float32
+ * log and power, so Roe agrees with oracle/fvm.py to within a few ulps of
+ * those per evaluation (also checked there). Python-float constants of the
+ * numpy code are formed in double and rounded to float32 where numpy's weak
+ * promotion does.
  */
 #include <math.h>
 #include <stdint.h>
@@ -56,6 +65,7 @@ typedef struct {
   float value[4];         /* inlet state (float32) */
   int32_t has_width, has_strength;
   double sponge_width, sponge_strength;
+  int32_t roe;            /* 1: Roe dissipation, 0: HLLC */
 } cf_opts;
 
 /* float32 forms of the Python-float expressions (numpy weak promotion) */
@@ -67,6 +77,8 @@ typedef struct {
   float gm1M2;    /* (gamma - 1) * M ** 2 */
   float i2;       /* 1 / M ** 2 */
   float d_gm1;    /* default gamma 1.4 - 1 */
+  float g2;       /* 2 * gamma */
+  int roe;
 } consts;
 
 static consts make_consts(const cf_opts* o) {
@@ -78,6 +90,8 @@ static consts make_consts(const cf_opts* o) {
   c.gm1M2 = (float)((o->gamma - 1.0) * (o->M * o->M));
   c.i2 = (float)(1.0 / (o->M * o->M));
   c.d_gm1 = (float)(1.4 - 1.0);
+  c.g2 = (float)(2.0 * o->gamma);
+  c.roe = o->roe;
   return c;
 }
 
@@ -288,7 +302,79 @@ static inline void hllc(const float* WL, const float* WR, float nx, float ny, co
   }
 }
 
-/* getFlux (euler_solver.py:376-420): central flux minus the HLLC dissipation. */
+/* Entropy variables (helper.py:57-70, with get_specific_entropy 48-55):
+ * s = log(P / rho^gamma), V = [(gamma - s) / (gamma - 1) - rho |u|^2 / (2P),
+ * rho u / P, rho v / P, -rho / P]. */
+static inline void entropy_vars(const float* W, const consts* c, float* V) {
+  const float rho = W[0];
+  const float u = FDIV(W[1], rho);
+  const float v = FDIV(W[2], rho);
+  const float P = (W[3] - 0.5f * rho * (u * u + v * v)) * c->gm1;
+  const float s = logf(FDIV(P, powf(rho, c->g)));
+  V[0] = FDIV(c->g - s, c->gm1) - FDIV(0.5f * rho * (u * u + v * v), P);
+  V[1] = FDIV(W[1], P);
+  V[2] = FDIV(W[2], P);
+  V[3] = FDIV(-rho, P);
+}
+
+/* Roe dissipation 0.5 R |Lambda S| R^T dv (euler_solver.py:145-199) with the
+ * Roe average of helper.py:101-127 (densities and pressures <= 0 clamped to
+ * 1e-6; pressures without the M^2 factor). */
+static inline void roe(const float* WL, const float* WR, float nx, float ny, const consts* c, float* out) {
+  float rs[2], us[2], vs[2], Es[2], Ps[2];
+  const float* Wk[2] = {WL, WR};
+  for (int q = 0; q < 2; ++q) {
+    float rho = Wk[q][0];
+    if (rho <= 0.0f) rho = 1e-6f;
+    const float u = FDIV(Wk[q][1], rho);
+    const float v = FDIV(Wk[q][2], rho);
+    const float E = Wk[q][3];
+    float P = (E - 0.5f * rho * (u * u + v * v)) * c->gm1;
+    if (P <= 0.0f) P = 1e-6f;
+    rs[q] = rho; us[q] = u; vs[q] = v; Es[q] = E; Ps[q] = P;
+  }
+  const float sL = sqrtf(rs[0]), sR = sqrtf(rs[1]);
+  const float r = sL * sR;
+  const float u = FDIV(sL * us[0] + sR * us[1], sL + sR);
+  const float v = FDIV(sL * vs[0] + sR * vs[1], sL + sR);
+  const float HL = FDIV(Es[0] + Ps[0], rs[0]);
+  const float HR = FDIV(Es[1] + Ps[1], rs[1]);
+  const float H = FDIV(sL * HL + sR * HR, sL + sR);
+  const float a = sqrtf(fabsf(c->gm1 * (H - 0.5f * (u * u + v * v))));
+  float Rm[4][4];
+  Rm[0][0] = 1.0f;
+  Rm[1][0] = u - a * nx;
+  Rm[2][0] = v - a * ny;
+  Rm[3][0] = H - a * (u * nx + v * ny);
+  Rm[0][1] = 1.0f;
+  Rm[1][1] = u;
+  Rm[2][1] = v;
+  Rm[3][1] = 0.5f * (u * u + v * v);
+  Rm[0][2] = 0.0f;
+  Rm[1][2] = -ny;
+  Rm[2][2] = nx;
+  Rm[3][2] = -u * ny + v * nx;
+  Rm[0][3] = 1.0f;
+  Rm[1][3] = u + a * nx;
+  Rm[2][3] = v + a * ny;
+  Rm[3][3] = H + a * (u * nx + v * ny);
+  const float un = u * nx + v * ny;
+  const float lam[4] = {un - a, un, un, un + a};
+  float VL[4], VR[4], dv[4];
+  entropy_vars(WR, c, VR);
+  entropy_vars(WL, c, VL);
+  for (int q = 0; q < 4; ++q) dv[q] = VR[q] - VL[q];
+  const float S[4] = {FDIV(r, c->g2), FDIV(c->gm1 * r, c->g), FDIV(r, c->g), FDIV(r, c->g2)};
+  float DRH[4];
+  for (int i = 0; i < 4; ++i) {
+    const float RH = ((Rm[0][i] * dv[0] + Rm[1][i] * dv[1]) + Rm[2][i] * dv[2]) + Rm[3][i] * dv[3];
+    DRH[i] = fabsf(lam[i] * S[i]) * RH;
+  }
+  for (int i = 0; i < 4; ++i)
+    out[i] = 0.5f * ((((Rm[i][0] * DRH[0] + Rm[i][1] * DRH[1]) + Rm[i][2] * DRH[2]) + Rm[i][3] * DRH[3]));
+}
+
+/* getFlux (euler_solver.py:376-420): central flux minus the dissipation. */
 static inline void face_flux(const float* WL, const float* WR, float nx, float ny, const consts* c, float* F) {
   float L[4], R[4];
   prim_of(WL, c->M2, c->gm1, L);
@@ -300,7 +386,10 @@ static inline void face_flux(const float* WL, const float* WR, float nx, float n
   const float fvL = L[0] * L[2] * unL + i2 * L[3] * ny, fvR = R[0] * R[2] * unR + i2 * R[3] * ny;
   const float fEL = (WL[3] + i2 * L[3]) * unL, fER = (WR[3] + i2 * R[3]) * unR;
   float D[4];
-  hllc(WL, WR, nx, ny, c, D);
+  if (c->roe)
+    roe(WL, WR, nx, ny, c, D);
+  else
+    hllc(WL, WR, nx, ny, c, D);
   F[0] = 0.5f * (frL + frR) - D[0];
   F[1] = 0.5f * (fuL + fuR) - D[1];
   F[2] = 0.5f * (fvL + fvR) - D[2];

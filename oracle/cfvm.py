@@ -3,11 +3,12 @@ oracle/libcfvm.so (oracle/cfvm.c, built by ``make -C oracle``).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may use it. It computes, in float32 and in parallel over cells (OpenMP),
-the same operations as oracle/fvm.py for HLLC with constant or MUSCL
+the same operations as oracle/fvm.py for HLLC or Roe with constant or MUSCL
 reconstruction: ``residual``, ``time_step_RK2_SSP`` (and ``srk2_steps``, n
-steps without leaving C) and ``get_dt``; the results are bit-identical to
-oracle/fvm.py (tests/test_oracle_c.py). Other fluxes, the entropy flag and
-float64 are not restated here.
+steps without leaving C) and ``get_dt``; with HLLC the results are
+bit-identical to oracle/fvm.py, with Roe they agree to a few ulps of the C
+library's logf / powf against numpy's (tests/test_oracle_c.py). Other fluxes,
+the entropy flag and float64 are not restated here.
 """
 from __future__ import annotations
 
@@ -33,7 +34,7 @@ class _Mesh(C.Structure):
 class _Opts(C.Structure):
     _fields_ = [("gamma", C.c_double), ("M", C.c_double), ("muscl", C.c_int32), ("value", C.c_float * 4),
                 ("has_width", C.c_int32), ("has_strength", C.c_int32), ("sponge_width", C.c_double),
-                ("sponge_strength", C.c_double)]
+                ("sponge_strength", C.c_double), ("roe", C.c_int32)]
 
 
 LIB_RCP = os.path.join(HERE, "libcfvm_rcp.so")
@@ -103,8 +104,8 @@ class CMesh:
 
 def _opts(kw):
     ft = F.flux_type(kw)
-    if ft not in ("hll", "hllc"):
-        raise ValueError(f"the C oracle restates HLLC only, not '{ft}'")
+    if ft not in ("hll", "hllc", "roe"):
+        raise ValueError(f"the C oracle restates HLLC and Roe only, not '{ft}'")
     if kw.get("entropy", False):
         raise ValueError("the C oracle does not restate the entropy flux")
     mode = F.reconstruction_mode(kw)
@@ -114,7 +115,7 @@ def _opts(kw):
     w, s = kw.get("sponge_width"), kw.get("sponge_strength")
     return _Opts(float(kw.get("gamma", 1.4)), float(kw.get("M", 1.0)), int(mode in F.RECON_MUSCL),
                  (C.c_float * 4)(*value.tolist()), int(w is not None), int(s is not None),
-                 0.0 if w is None else float(w), 0.0 if s is None else float(s))
+                 0.0 if w is None else float(w), 0.0 if s is None else float(s), int(ft == "roe"))
 
 
 def _cm(mesh_or_cm, kw):

@@ -3,9 +3,10 @@ h = 0.025 (22k and 28k cells), M in {0.9, 1.1, 2.0}, alpha = 2 deg, HLLC +
 MUSCL, SSP-RK2 at CFL 0.6 (0.45 in the transonic band), from freestream. The
 six cases run as one GPU batch for 100 steps and are compared cell by cell
 with the C restatement of the oracle (oracle/cfvm.c, float32, bit-identical
-to oracle/fvm.py) fed the same time step: every primitive within 1e-5 of
-max(|p|, freestream scale), the bar of the hot path; the time step against
-the oracle's get_dt within 2e-7."""
+to oracle/fvm.py for HLLC) fed the same time step: every primitive within
+1e-5 of max(|p|, freestream scale), the bar of the hot path; the time step
+against the oracle's get_dt within 2e-7. The same six cases with Roe (the
+other half of the headline campaign's cases) are held to the same bar."""
 import numpy as np
 import pytest
 
@@ -17,13 +18,13 @@ MACHS, AOA, STEPS = (0.9, 1.1, 2.0), 2.0, 100
 _ORACLE = {}
 
 
-def _cases():
+def _cases(flux="HLLC"):
     from eulersr_amd import main as M
     specs, inits, cfgs = [], [], []
     for geom in ("diamond", "naca0012"):
         mesh = mesh_of(geom, 0.025)
         for mach in MACHS:
-            cfg = base_cfg(Mach=mach, aoa=AOA, case=geom, flux="HLLC", reconstruction="MUSCL", CFL=0.6)
+            cfg = base_cfg(Mach=mach, aoa=AOA, case=geom, flux=flux, reconstruction="MUSCL", CFL=0.6)
             W, inlet = M.initialize(mesh, cfg)
             specs.append(M.case_spec(mesh, inlet, cfg))
             inits.append(W)
@@ -31,24 +32,26 @@ def _cases():
     return specs, inits, cfgs
 
 
-def _oracle(i, sp, W0, dt):
+def _oracle(i, sp, W0, dt, flux="HLLC"):
     """100 SSP-RK2 steps of case i on the C oracle (cached across launches)."""
     from oracle import cfvm as CF
     from oracle import fvm as F
-    key = (i, float(dt))
+    key = (i, float(dt), flux)
     if key not in _ORACLE:
         cm = CF.CMesh(F.omesh(sp.mesh, np.float32))
-        kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=sp.kw["value"], entropy=False)
+        kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux=flux, value=sp.kw["value"], entropy=False)
         _ORACLE[key] = (CF.get_dt(W0, cm, CFL=sp.cfl), CF.srk2_steps(W0, cm, np.float32(dt), STEPS, **kw))
     return _ORACLE[key]
 
 
-@pytest.mark.parametrize("launch", [("cell", 64, True), ("face", 64, True)])
-def test_c2_cases_100_steps_within_1e5(launch):
+@pytest.mark.parametrize("flux,launch", [("HLLC", ("cell", 64, True)), ("HLLC", ("face", 64, True)),
+                                         ("ROE", ("cell", 64, True)), ("ROE", ("face", 64, True))])
+def test_c2_cases_100_steps_within_1e5(flux, launch):
     from eulersr_amd.solve import BatchSolver
     from oracle import fvm as F
-    specs, inits, cfgs = _cases()
-    print(f"launch {launch}: {[sp.mesh.n_cells for sp in specs]} cells, cfl {[sp.cfl for sp in specs]}", flush=True)
+    specs, inits, cfgs = _cases(flux)
+    print(f"flux {flux} launch {launch}: {[sp.mesh.n_cells for sp in specs]} cells, cfl {[sp.cfl for sp in specs]}",
+          flush=True)
     bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=1, max_steps=STEPS,
                      launch=launch)
     try:
@@ -63,7 +66,7 @@ def test_c2_cases_100_steps_within_1e5(launch):
     for i, (sp, W0, cfg) in enumerate(zip(specs, inits, cfgs)):
         dt = reps[i]["dt"]
         assert reps[i]["steps_run"] == STEPS
-        dt_or, Wr = _oracle(i, sp, W0, dt)
+        dt_or, Wr = _oracle(i, sp, W0, dt, flux)
         assert abs(float(dt) - float(dt_or)) <= 2e-7 * float(dt_or), (float(dt), float(dt_or))
         pg = F.getPrimitive(states[i].astype(np.float64))
         pr = F.getPrimitive(Wr.astype(np.float64))
