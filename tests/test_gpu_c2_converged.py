@@ -1,14 +1,15 @@
 """Converged CD/CL of the single-case GPU configuration against the oracle:
 diamond (half-angle 5 deg, reference box) at M = 2.0 and NACA0012 at M = 0.9
 (transonic: the CFL clamp to 0.45 of main.py:117-120 applies), alpha = 2
-deg, h = 0.025, HLLC + MUSCL, SSP-RK2, CFL 0.6. Both run in one batched GPU
+deg, h = 0.025, HLLC + MUSCL, SSP-RK2, CFL 0.6, and the diamond case again
+with Roe (the headline campaign's other flux). They run in one batched GPU
 solve until the stationarity residual stays below 1e-5 for 3 consecutive
 checks every 5 steps (the convergence defaults of eval/live_fvm.py:37-47;
 main.py:141-199 semantics: the state freezes when the streak completes).
-The C restatement of the oracle (oracle/cfvm.c, float32, bit-identical to
-oracle/fvm.py) then advances the same freestream state the same number of
-steps with the same dt, and CD and CL (helper.py:366-415, FP64 wall sums)
-must agree within 1e-4."""
+The C restatement of the oracle (oracle/cfvm.c, float32; with HLLC
+bit-identical to oracle/fvm.py) then advances the same freestream state the
+same number of steps with the same dt, and CD and CL (helper.py:366-415,
+FP64 wall sums) must agree within 1e-4."""
 import numpy as np
 import pytest
 
@@ -16,7 +17,7 @@ from conftest import base_cfg, mesh_of
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("diamond", 2.0), ("naca0012", 0.9)]
+CASES = [("diamond", 2.0, "HLLC"), ("naca0012", 0.9, "HLLC"), ("diamond", 2.0, "ROE")]
 
 
 def test_c2_converged_forces_match_oracle():
@@ -25,16 +26,16 @@ def test_c2_converged_forces_match_oracle():
     from oracle import cfvm as CF
     from oracle import fvm as F
     specs, inits, meta = [], [], []
-    for geom, mach in CASES:
+    for geom, mach, flux in CASES:
         mesh = mesh_of(geom, 0.025)
-        cfg = base_cfg(Mach=mach, aoa=2.0, case=geom, CFL=0.6)
+        cfg = base_cfg(Mach=mach, aoa=2.0, case=geom, CFL=0.6, flux=flux)
         W0, inlet = M.initialize(mesh, cfg)
         sp = M.case_spec(mesh, inlet, cfg)
         sp.tf = 1.0e6
         specs.append(sp)
         inits.append(W0)
         meta.append((geom, mach, mesh, inlet, cfg))
-    print(f"case: C2 converged, {[(g, m, x[2].n_cells) for (g, m), x in zip(CASES, meta)]}", flush=True)
+    print(f"case: C2 converged, {[(g, m, f, x[2].n_cells) for (g, m, f), x in zip(CASES, meta)]}", flush=True)
     bs = BatchSolver(specs, scheme="SRK2", threshold=1e-5, check_every=5, patience=3, max_steps=30000)
     try:
         for i, W in enumerate(inits):
@@ -48,12 +49,12 @@ def test_c2_converged_forces_match_oracle():
         bs.close()
     for (geom, mach, mesh, inlet, cfg), rep, Wg, W0 in zip(meta, reps, states, inits):
         steps = rep["steps_run"]
-        print(f"GPU {geom} M={mach}: converged={rep['converged']} stopping_step={rep['stopping_step']} "
-              f"steps_run={steps} dt={rep['dt']} rel={rep['last_rel']:.3e} CD={rep['cd']:.6f} CL={rep['cl']:.6f}",
+        print(f"GPU {geom} M={mach} {cfg['flux']}: converged={rep['converged']} "
+              f"stopping_step={rep['stopping_step']} steps_run={steps} dt={rep['dt']} rel={rep['last_rel']:.3e} CD={rep['cd']:.6f} CL={rep['cl']:.6f}",
               flush=True)
         assert rep["converged"] and steps > 1000
         om = F.omesh(mesh, np.float32)
-        kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux="HLLC", value=inlet, entropy=False)
+        kw = dict(gamma=1.4, M=1.0, reconstruction="MUSCL", flux=cfg["flux"], value=inlet, entropy=False)
         Wr = CF.srk2_steps(W0, CF.CMesh(om), np.float32(rep["dt"]), steps, **kw)
         U = mach * np.sqrt(1.4)
         cd, cl = F.forces(Wr, om, 1.0, U, mesh.metadata["obstacle_length"])
