@@ -109,6 +109,7 @@ SIGNATURES = {
     "esr_solver_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "esr_solver_time_kernels": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
+    "esr_solver_time_kernel_runs": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_void_p]),
     "esr_check_device_math": (C.c_int, [C.POINTER(C.c_uint64), C.c_void_p]),
     "esr_set_stage_path": (C.c_int, [C.c_int32]),
     "esr_solver_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]),

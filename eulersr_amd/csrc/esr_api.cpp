@@ -14,6 +14,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -1336,6 +1337,52 @@ esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void
       }
     }
     for (int k = 0; k < 4; ++k) ms[k] = acc[k] / reps;
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    if (s->prepared) launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
+  });
+}
+
+esr_status esr_solver_time_kernel_runs(esr_solver* s, int32_t reps, double* ms, void* stream) {
+  return guard([&] {
+    if (!s || reps <= 0 || !ms) throw Error(ESR_ERR_VALUE, "bad arguments");
+    hipStream_t st = S(stream);
+    Engine& e = s->eng;
+    launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);
+    // each launch set reads what the one before it in a stage wrote (rec,
+    // fx) and rewrites its own outputs, so repeating it is idempotent
+    auto recon = [&] { launch_recon(e.rec_all.ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(), st); };
+    auto face = [&] {
+      for (auto& cl : e.classes) launch_face(cl->ls, e.cases(), e.W0.as<float4>(), e.rec.as<float4>(),
+                                             e.fx.as<float4>(), st);
+    };
+    auto update = [&] {
+      launch_update(e.cell_all.ls, MODE_S1, e.local_dt, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(),
+                    e.fx.as<float4>(), e.W1.as<float4>(), e.acc.as<float4>(), st);
+    };
+    auto cell = [&] {
+      for (auto& cl : e.classes)
+        launch_cell(cl->ls, MODE_S1, e.cases(), e.W0.as<float4>(), e.W0.as<float4>(), e.rec.as<float4>(),
+                    e.W1.as<float4>(), e.acc.as<float4>(), st);
+    };
+    const std::function<void()> kern[4] = {recon, face, update, cell};
+    recon();
+    face();
+    update();
+    cell();  // code objects loaded, buffers valid
+    hipEvent_t ev[2];
+    for (auto& x : ev) hip_check(hipEventCreateWithFlags(&x, hipEventDisableSystemFence), "event");
+    for (int k = 0; k < 4; ++k) {
+      // the kernels before it in its stage, so its inputs are the stage's
+      if (k >= 1) recon();
+      if (k == 2) face();
+      hip_check(hipEventRecord(ev[0], st), "record");
+      for (int32_t r = 0; r < reps; ++r) kern[k]();
+      hip_check(hipEventRecord(ev[1], st), "record");
+      hip_check(hipEventSynchronize(ev[1]), "event sync");
+      float t = 0.f;
+      hip_check(hipEventElapsedTime(&t, ev[0], ev[1]), "elapsed");
+      ms[k] = (double)t / reps;
+    }
     for (auto& x : ev) (void)hipEventDestroy(x);
     if (s->prepared) launch_set_active(e.n_cases, e.cases(), e.loops_d.as<CaseLoop>(), 0, st);
   });

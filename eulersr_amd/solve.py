@@ -208,6 +208,15 @@ class BatchSolver:
         L.check(L.lib().esr_solver_time_kernels(self.handle, int(reps), ms, stream_ptr()), "esr_solver_time_kernels")
         return tuple(ms)
 
+    def time_kernel_runs(self, reps=50):
+        """Average ms per launch set of the same four kernels, each launched
+        `reps` times back to back between two HIP events (the kernel's
+        duration plus only the gap between queued launches)."""
+        ms = (C.c_double * 4)()
+        L.check(L.lib().esr_solver_time_kernel_runs(self.handle, int(reps), ms, stream_ptr()),
+                "esr_solver_time_kernel_runs")
+        return tuple(ms)
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             L.lib().esr_solver_destroy(self.handle)

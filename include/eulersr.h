@@ -302,6 +302,14 @@ esr_status esr_solver_info(const esr_solver* s, int64_t* cell_items, int64_t* gr
  * before the kernel; all are idempotent (they rewrite scratch buffers and
  * the stage buffer only). ms holds 4 entries. */
 esr_status esr_solver_time_kernels(esr_solver* s, int32_t reps, double* ms, void* stream);
+/* The same four kernels each launched `reps` times back to back between two
+ * HIP events on `stream` (after the kernels before it in its stage, so its
+ * inputs are a stage's): ms[k] = the average per launch, the kernel's
+ * duration plus only the gap between two queued launches, where
+ * esr_solver_time_kernels' intervals also hold each launch's dispatch after
+ * an event. Idempotent as esr_solver_time_kernels. No counterpart in the
+ * reference. */
+esr_status esr_solver_time_kernel_runs(esr_solver* s, int32_t reps, double* ms, void* stream);
 
 /* Exhaustive self-check of the device arithmetic: the square root and the
  * natural logarithm that the flux kernels evaluate (sound speeds, Roe
