@@ -85,3 +85,31 @@ def test_set_launch_rejects_bad_values():
     with pytest.raises(KeyError):
         bs.set_launch("edge", 64)
     bs.close()
+
+
+def test_kernel_timings_leave_the_state_alone():
+    """esr_solver_time_kernels and esr_solver_time_kernel_runs return a
+    positive time per stage kernel and leave every case's state and the
+    solve that follows bit for bit as they were."""
+    from eulersr_amd.solve import BatchSolver
+    specs, inits = _cases()
+    ref, _ = _run(specs, inits, launch=("cell", 64, True), steps=20)
+    bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=5, patience=1, max_steps=20)
+    for i, W in enumerate(inits):
+        bs.set_state(i, W)
+    bs.prepare()
+    bs.set_launch("cell", 64, True)
+    bs.run(10)
+    before = [bs.get_state(i).cpu().numpy() for i in range(len(specs))]
+    a = bs.time_kernels(5)
+    b = bs.time_kernel_runs(5)
+    print(f"in stage {a}, back to back {b}", flush=True)
+    assert all(x > 0 and np.isfinite(x) for x in a + b)
+    after = [bs.get_state(i).cpu().numpy() for i in range(len(specs))]
+    for x, y in zip(before, after):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
+    bs.run()
+    got = [bs.get_state(i).cpu().numpy() for i in range(len(specs))]
+    bs.close()
+    for x, y in zip(got, ref):
+        assert np.array_equal(x.view(np.uint32), y.view(np.uint32))
