@@ -144,16 +144,18 @@ def traffic_file(tag):
 def measured_traffic(kernel, tuning, tag):
     """HBM bytes per launch set of `kernel` from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (per-dispatch means,
-    calibrated against a 1 GiB copy), when that summary was taken with the
-    same launch configuration; else None. Kernels templated on the flux
-    (k_face, k_cell) are dispatched once per flux class (their instances are
-    summed); k_recon and k_update are one dispatch over every group."""
+    calibrated against a 1 GiB copy), when that summary was taken on the same
+    stage path (the block size and order move which work-items share a
+    block, not the bytes a launch needs: `traffic_launch` names the profiled
+    one); else None. Kernels templated on the flux (k_face, k_cell) are
+    dispatched once per flux class (their instances are summed); k_recon and
+    k_update are one dispatch over every group."""
     path = traffic_file(tag)
     if not os.path.exists(path):
         return None
     from eulersr_amd.profiling import read_traffic, read_traffic_label
-    want = f"launch {tuning['path']},{tuning['block']},{int(bool(tuning['xcd_order']))}"
-    if read_traffic_label(path) != want:
+    label = read_traffic_label(path) or ""
+    if not label.startswith(f"launch {tuning['path']},"):
         return None
     pre = PROFILE_PREFIX[kernel]
     hits = [fb + wb for name, (fb, wb) in read_traffic(path).items()
@@ -161,6 +163,15 @@ def measured_traffic(kernel, tuning, tag):
     if not hits:
         return None
     return sum(hits) if pre.endswith("<") else hits[0]
+
+
+def traffic_launch(tag):
+    """The launch configuration the committed traffic summary was profiled at."""
+    path = traffic_file(tag)
+    if not os.path.exists(path):
+        return None
+    from eulersr_amd.profiling import read_traffic_label
+    return read_traffic_label(path)
 
 
 def kernel_table(sizes, ms, runs=None):
@@ -1004,6 +1015,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": kern[dom]["gbs_runs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": kern[dom]["gbs_runs"] / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dom, tuning, f"{args.slots}slot_h{args.h}"),
+                     "traffic_launch": traffic_launch(f"{args.slots}slot_h{args.h}"),
                      "kernel": dom, "stage_path": tuning["path"], "bytes_per_launch": kern[dom]["bytes_per_launch"],
                      "us_per_launch": kern[dom]["us_per_launch_runs"],
                      "us_per_launch_in_stage": kern[dom]["us_per_launch"],

@@ -42,7 +42,7 @@ def test_occupancy_from_a_campaign_trace():
     assert bench.occupancy(trace[:1], slots=4) is None
 
 
-def test_measured_traffic_needs_the_profiled_launch(tmp_path, monkeypatch):
+def test_measured_traffic_needs_the_profiled_stage_path(tmp_path, monkeypatch):
     path = tmp_path / "traffic_bench_x.txt"
     path.write_text("# launch cell,64,1\n# kernel fetch write\nk_cell<1, 100, 0> 100 20\nk_cell<1, 4, 0> 10 2\n"
                     "k_recon 50 80\nk_update<2> 30 10\n")
@@ -52,5 +52,9 @@ def test_measured_traffic_needs_the_profiled_launch(tmp_path, monkeypatch):
     assert bench.measured_traffic("k_cell<stage1>", pinned, "x") == 132.0
     assert bench.measured_traffic("k_recon", pinned, "x") == 130.0
     assert bench.measured_traffic("k_update<stage1>", pinned, "x") == 40.0
-    # another launch configuration than the profiled one: no traffic figure
-    assert bench.measured_traffic("k_cell<stage1>", dict(pinned, block=128), "x") is None
+    # another block size or order on the same path: the profiled bytes, with
+    # the launch they were profiled at named
+    assert bench.measured_traffic("k_cell<stage1>", dict(pinned, block=256, xcd_order=False), "x") == 132.0
+    assert bench.traffic_launch("x") == "launch cell,64,1"
+    # another stage path: no traffic figure
+    assert bench.measured_traffic("k_cell<stage1>", dict(pinned, path="face"), "x") is None
