@@ -17,7 +17,7 @@ one-step probe and its last 16 check residuals) is the case's threshold; a
 solve has converged when the stationarity residual stays below it for 3
 consecutive checks (every 5 steps), within a cap of 40,000 steps per case.
 Set-up outside the timed region (times reported): the reference fields, per
-geometry 55 solver-generated LR (h = 0.1) / HR training pairs (none of them
+geometry 216 solver-generated LR (h = 0.1) / HR training pairs (none of them
 a test case) and the training of one warm-start network per geometry.
 
 A "step" of this benchmark is one warm-start campaign over the 1,024 cases:
@@ -190,9 +190,24 @@ def kernel_table(sizes, ms, runs=None):
 
 
 GEOMETRIES = ("diamond", "naca0012", "naca2412", "naca4412", "naca0015")
-# training grid of the warm-start networks (no test case is on it)
+# training grid of the warm-start networks: a coarse grid on the test
+# cases' grid (test cases that fall on it are moved off, ragged_grid) and a
+# finer one whose angles lie between the test grid's 0.5 deg steps, so no
+# test case can be a training case
 TRAIN_MACHS = (0.7, 0.8, 0.95, 1.05, 1.2, 1.4, 1.6, 1.8, 2.2, 2.5, 2.8)
 TRAIN_AOAS = (-4.0, -1.0, 1.0, 3.0, 5.0)
+TRAIN_MACHS_FINE = tuple(round(0.75 + 0.1 * i, 2) for i in range(23))
+TRAIN_AOAS_FINE = (-4.75, -3.25, -1.75, -0.25, 1.25, 2.75, 4.25)
+
+
+def train_grid():
+    """(M, alpha) of the solver-generated LR/HR training pairs, 216 per
+    geometry: 55 on the coarse grid and 161 on the fine one (measured in one
+    process on the headline campaign against the coarse grid alone: 3.6 %
+    fewer HR steps, 7.5 % less HR time, 435 instead of 421 cases within the
+    |dCD| bar)."""
+    return ([(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
+            + [(m, a) for m in TRAIN_MACHS_FINE for a in TRAIN_AOAS_FINE])
 SMALL_MACHS, SMALL_AOA = (0.9, 1.1, 2.0), 2.0
 
 
@@ -293,10 +308,10 @@ def cd_history(args):
     from eulersr_amd import pipeline as P
     from eulersr_amd.solve import BatchSolver
     meshes = build_meshes(GEOMETRIES, args.h, args.h_lr)
-    train_grid = [(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
+    grid_t = train_grid()
     starter = P.WarmStarter()
     for g, (lm, hm) in meshes.items():
-        pairs = P.training_pairs(g, lm, hm, train_grid)
+        pairs = P.training_pairs(g, lm, hm, grid_t)
         kmap = P.WS.KnnMap(lm, hm)
         net, _ = P.train(kmap, pairs, steps=args.train_steps, batch=args.train_batch, seed=0)
         starter.add(g, lm, hm, net=net, kmap=kmap)
@@ -590,10 +605,13 @@ def small_config(meshes, starter, args):
         hs, hi = P.make_cases(hm, g, grid)
         ref, _ = P.fixed_time(hs, hi)
         cases += P.warm_cases(g, lm, hm, grid, ref)
-    starter.run(cases, mode="mlp", max_steps=20)
-    warm = starter.run(cases, mode="mlp", max_steps=args.max_steps)
-    cold = starter.run(cases, mode="cold", max_steps=args.max_steps)
-    idw = starter.run(cases, mode="idw", max_steps=args.max_steps)
+    # the headline's protocol: continuous batching with every case in flight
+    # (stall restarts and hedged fallbacks included)
+    n = len(cases)
+    starter.campaign(cases, mode="mlp", slots=n, max_steps=20, lr_max_steps=20)
+    warm = starter.campaign(cases, mode="mlp", slots=n, max_steps=args.max_steps)
+    cold = starter.campaign(cases, mode="cold", slots=n, max_steps=args.max_steps)
+    idw = starter.campaign(cases, mode="idw", slots=n, max_steps=args.max_steps)
     st, sc, single = plain_single(cases, args.max_steps)
     single_rate = sc / st * 3600.0 if st > 0 else 0.0
     kern, sizes, tuning = stage_kernels(cases, args.kernel_reps, P.LAUNCH)
@@ -603,8 +621,8 @@ def small_config(meshes, starter, args):
                idw_batched_cases_per_hour=rate(idw), plain_single_case_cases_per_hour=single_rate,
                speedup_vs_plain_single_case=rate(warm) / single_rate if single_rate else None,
                pipeline_split_s={"lr": warm["lr_s"], "inference": warm["infer_s"], "hr": warm["hr_s"]},
-               steps_per_case={"mlp": [x["stopping_step"] for x in warm["reports"]],
-                               "idw": [x["stopping_step"] for x in idw["reports"]],
+               steps_per_case={"mlp": [x["steps_total"] for x in warm["reports"]],
+                               "idw": [x["steps_total"] for x in idw["reports"]],
                                "plain_single": [x["stopping_step"] for x in single]},
                abs_dcd_vs_plain=[abs(w["cd"] - p["cd"]) for w, p in zip(warm["reports"], single)],
                tuning=tuning, stage_kernels=kern, dominant_kernel=dom,
@@ -812,20 +830,20 @@ def main():
         log(f"  {g}: LR {lm.n_cells} cells, HR {hm.n_cells} cells")
 
     # warm-start network per geometry, trained on solver-generated pairs
-    train_grid = [(m, a) for m in TRAIN_MACHS for a in TRAIN_AOAS]
+    grid_t = train_grid()
     starter = P.WarmStarter()
     training = {}
     for g, (lm, hm) in meshes.items():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pairs = P.training_pairs(g, lm, hm, train_grid, max_steps=cap)
+        pairs = P.training_pairs(g, lm, hm, grid_t, max_steps=cap)
         torch.cuda.synchronize()
         t_data = time.perf_counter() - t0
         kmap = P.WS.KnnMap(lm, hm)
         net, info = P.train(kmap, pairs, steps=args.train_steps, batch=args.train_batch, seed=0)
-        info.update(data_s=t_data, cases=len(train_grid))
+        info.update(data_s=t_data, cases=len(grid_t))
         training[g] = info
-        log(f"  {g}: {len(train_grid)} LR/HR pairs in {t_data:.1f}s; trained {info['steps']} steps in "
+        log(f"  {g}: {len(grid_t)} LR/HR pairs in {t_data:.1f}s; trained {info['steps']} steps in "
             f"{info['train_s']:.1f}s, loss {info['loss_first']:.3e} -> {info['loss_last']:.3e} "
             f"(IDW-only {info['idw_mse']:.3e})")
         del pairs
@@ -1005,7 +1023,7 @@ def main():
                                "(diamond, NACA0012/2412/4412/0015), (M, aoa) from M in [0.7,3.0] x aoa in [-5,5], "
                                "half HLLC half Roe, MUSCL, SSP-RK2, CFL 0.6 (0.45 transonic); warm start from an "
                                f"LR h={args.h_lr} fixed-time solve through kNN(6)/IDW features and a 4x256 BF16 "
-                               "MLP per geometry trained on 55 solver-generated LR/HR pairs",
+                               "MLP per geometry trained on 216 solver-generated LR/HR pairs",
                    "cases": args.cases, "h": args.h, "h_lr": args.h_lr, "cells_total": int(sum(cells))},
         "steps_requested": args.steps, "warmup_requested": args.warmup,
         "warmup_note": "one 8-case code-object and launch-tuning pass; campaigns keep no state between runs",

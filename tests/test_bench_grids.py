@@ -22,6 +22,18 @@ def test_learned_split_is_disjoint_on_the_grid_and_seeded():
     assert bench.learned_grid(2000, 64) == grids          # default_rng(1): reproducible
 
 
+def test_training_grid_holds_no_test_case():
+    """The coarse training grid lies on the test grid (test cases on it are
+    moved off); the fine one's angles lie between the test grid's steps."""
+    grid = bench.train_grid()
+    assert len(grid) == len(set(grid)) == 216
+    tests = {(m, a) for _, m, a, _ in bench.ragged_grid(1024)}
+    assert not tests & set(grid)
+    fine = [(m, a) for m in bench.TRAIN_MACHS_FINE for a in bench.TRAIN_AOAS_FINE]
+    assert all(a not in AOAS for _, a in fine)
+    assert all(0.7 <= m <= 3.0 and -5.0 <= a <= 5.0 for m, a in grid)
+
+
 def test_ragged_grid_stays_off_the_training_grid():
     grid = bench.ragged_grid(1024)
     assert len(grid) == 1024
