@@ -55,8 +55,11 @@ void check_rows(const esr_mlp* m, int64_t rows) {
     throw Error(ESR_ERR_VALUE, "rows must be in [1, max_rows] (" + std::to_string(m->max_rows) + ")");
 }
 
-// forward pass over `rows` rows of X (bf16, rows x w[0]); output in Z[L-1]
-void forward(esr_mlp* m, int64_t rows, const uint16_t* X, hipStream_t st) {
+// forward pass over `rows` rows of X (bf16, rows x w[0]); output in Z[L-1],
+// or in Y (rows x w[L], fp32) when given. Training keeps every layer's fp32
+// pre-activation Z[l] for the backward pass; inference (Y given) writes only
+// the hidden layers' bf16 activations, which the next layer reads.
+void forward(esr_mlp* m, int64_t rows, const uint16_t* X, hipStream_t st, float* Y = nullptr) {
   const uint16_t* in = X;
   for (int l = 0; l < m->L; ++l) {
     GemmArgs g{};
@@ -70,9 +73,9 @@ void forward(esr_mlp* m, int64_t rows, const uint16_t* X, hipStream_t st) {
     g.sb_k = 1;
     g.sb_j = m->w[l];
     g.bias = m->P.as<float>() + m->boff[l];
-    g.C32 = m->Z[l]->as<float>();
-    g.ldc32 = m->w[l + 1];
     const bool hidden = l < m->L - 1;
+    g.C32 = Y ? (hidden ? nullptr : Y) : m->Z[l]->as<float>();
+    g.ldc32 = m->w[l + 1];
     g.C16 = hidden ? m->H[l]->as<uint16_t>() : nullptr;
     g.ldc16 = m->w[l + 1];
     g.relu = hidden ? 1 : 0;
@@ -237,10 +240,7 @@ esr_status esr_mlp_forward(esr_mlp* m, int64_t rows, const uint16_t* X, float* Y
     const int32_t nin = m->w[0], nout = m->w[m->L];
     for (int64_t r0 = 0; r0 < rows; r0 += m->max_rows) {
       const int64_t r = std::min(m->max_rows, rows - r0);
-      forward(m, r, X + r0 * nin, st);
-      hip_check(hipMemcpyAsync(Y + r0 * nout, m->Z[m->L - 1]->p, (size_t)r * nout * sizeof(float),
-                               hipMemcpyDeviceToDevice, st),
-                "output copy");
+      forward(m, r, X + r0 * nin, st, Y + r0 * nout);
     }
   });
 }
