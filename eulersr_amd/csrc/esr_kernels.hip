@@ -278,33 +278,40 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
   const float4* Wg = Ws + G->st_off;
   float4* rg = rec + G->rec_off;
 
+  // Every load is issued before the arithmetic: the face records, the
+  // neighbours' states (a boundary face reads the cell's own state, in
+  // bounds, and replaces it by the ghost state below), the LSQ weights and
+  // the case's inlet state. Loaded face by face, each neighbour load would
+  // wait behind the previous face's branch.
   const float4 Wi = ld(Wg, it.e);
-  const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
-
-  Prim pr[3];
-  bool bnd[3];
+  uint32_t f[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const uint32_t f = ld(m.fref, kn(k, n, cell));
-    float4 WR;
-    if (fref_is_bnd(f)) {
-      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? ld(m.nrm, kn(k, n, cell)) : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Wg, sel(G, (int32_t)(f & FREF_BIDX), it.c));
-      bnd[k] = true;
-    } else {
-      WR = ld(Wg, sel(G, (int32_t)(f & FREF_IDX), it.c));
-      bnd[k] = false;
-    }
-    pr[k] = prim_of(WR, ph.M2, ph.gm1);
-  }
-
+  for (int k = 0; k < 3; ++k) f[k] = ld(m.fref, kn(k, n, cell));
+  float4 Wn[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) Wn[k] = ld(Wg, sel(G, fref_is_bnd(f[k]) ? cell : (int32_t)(f[k] & FREF_IDX), it.c));
   // weighted least squares (helper.py:136-153): grad = A^-1 sum_k w_k dW_k dl_k
   const float2 w0 = ld(m.wdl, kn(0, n, cell)), w1 = ld(m.wdl, kn(1, n, cell)), w2 = ld(m.wdl, kn(2, n, cell));
   const float4 ai = ld(m.ainv, (uint32_t)cell);
   float2 dm[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) dm[k] = ld(m.dxm, kn(k, n, cell));
+  const float4 value = cs.value;
+
+  const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
+  Prim pr[3];
+  bool bnd[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    float4 WR = Wn[k];
+    bnd[k] = fref_is_bnd(f[k]);
+    if (bnd[k]) {
+      const uint32_t kind = (f[k] >> FREF_KIND_SHIFT) & 7u;
+      const float2 nk = (kind == G_WALL) ? ld(m.nrm, kn(k, n, cell)) : make_float2(0.f, 0.f);
+      WR = ghost_state(kind, Wi, nk, value, m.inlet_subsonic, Wg, sel(G, (int32_t)(f[k] & FREF_BIDX), it.c));
+    }
+    pr[k] = prim_of(WR, ph.M2, ph.gm1);
+  }
 
   const float qi[4] = {pi.r, pi.u, pi.v, pi.P};
   float D[3][4];
@@ -377,10 +384,11 @@ __device__ __forceinline__ float sound(const Prim& q, const Phys& ph) {
   return fsqrt(fabsf(ph.g * q.P * frcp(q.r))) * ph.invM;
 }
 
-// HLLC flux F_HLLC (euler_solver.py:273-372); FL, FR receive the physical
-// fluxes of the two states.
+// HLLC flux F_HLLC (euler_solver.py:273-372). Every branch of the scheme
+// takes the physical flux of one side K (left for SL >= 0 or the left star
+// state, right otherwise), so only that side's physical flux is formed.
 __device__ __forceinline__ void hllc_flux(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
-                                          float ny, const Phys& ph, float F[4], float FL[4], float FR[4]) {
+                                          float ny, const Phys& ph, float F[4]) {
   const float unL = L.u * nx + L.v * ny;
   const float unR = R.u * nx + R.v * ny;
   const float tx = -ny, ty = nx;
@@ -392,40 +400,32 @@ __device__ __forceinline__ void hllc_flux(const float4 WL, const Prim& L, const 
   float SR = fmaxf(unL + aL, unR + aR);
   SL = (SL == SR) ? SL - 1e-6f : SL;
   SR = (SL == SR) ? SR + 1e-6f : SR;
-  phys_flux(WL, L, unL, nx, ny, ph.invM2, FL);
-  phys_flux(WR, R, unR, nx, ny, ph.invM2, FR);
   float den = L.r * (SL - unL) - R.r * (SR - unR);
   den = (den == 0.0f) ? 1e-12f : den;
   const float Ss = fdiv(R.P - L.P + L.r * unL * (SL - unL) - R.r * unR * (SR - unR), den);
-  if (SL >= 0.0f) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) F[v] = FL[v];
-  } else {
+  // SL >= 0: F_L; left star state (SL <= 0 <= S*): F_L + SL (U*_L - U_L);
+  // right star state (S* <= 0 <= SR): F_R + SR (U*_R - U_R); otherwise F_R
+  const bool left = (SL <= 0.0f && Ss >= 0.0f);
+  const bool takeL = (SL >= 0.0f) || left;
+  const bool star = !(SL >= 0.0f) && (left || (Ss <= 0.0f && SR >= 0.0f));
+  const Prim& K = takeL ? L : R;
+  const float4 WK = takeL ? WL : WR;
+  const float unK = takeL ? unL : unR;
+  phys_flux(WK, K, unK, nx, ny, ph.invM2, F);
+  if (star) {
     const float PsL = L.P + L.r * (SL - unL) * (Ss - unL);
     const float PsR = R.P + R.r * (SR - unR) * (Ss - unR);
     const float Ps = 0.5f * (PsL + PsR);
-    // left or right star state, selected without duplicating the arithmetic
-    const bool left = (SL <= 0.0f && Ss >= 0.0f);
-    const bool right = !left && (Ss <= 0.0f && SR >= 0.0f);
-    if (left || right) {
-      const float S = left ? SL : SR;
-      const Prim& K = left ? L : R;
-      const float4 WK = left ? WL : WR;
-      const float unK = left ? unL : unR;
-      const float utK = left ? utL : utR;
-      const float* FK = left ? FL : FR;
-      const float idn = frcp(S - Ss);
-      const float rs = K.r * (S - unK) * idn;
-      const float ux = Ss * nx + utK * tx, uy = Ss * ny + utK * ty;
-      const float Es = ((S - unK) * WK.w - K.P * unK + Ps * Ss) * idn;
-      F[0] = FK[0] + S * (rs - WK.x);
-      F[1] = FK[1] + S * (rs * ux - WK.y);
-      F[2] = FK[2] + S * (rs * uy - WK.z);
-      F[3] = FK[3] + S * (Es - WK.w);
-    } else {
-#pragma unroll
-      for (int v = 0; v < 4; ++v) F[v] = FR[v];
-    }
+    const float S = left ? SL : SR;
+    const float utK = left ? utL : utR;
+    const float idn = frcp(S - Ss);
+    const float rs = K.r * (S - unK) * idn;
+    const float ux = Ss * nx + utK * tx, uy = Ss * ny + utK * ty;
+    const float Es = ((S - unK) * WK.w - K.P * unK + Ps * Ss) * idn;
+    F[0] = F[0] + S * (rs - WK.x);
+    F[1] = F[1] + S * (rs * ux - WK.y);
+    F[2] = F[2] + S * (rs * uy - WK.z);
+    F[3] = F[3] + S * (Es - WK.w);
   }
 }
 
@@ -433,7 +433,9 @@ __device__ __forceinline__ void hllc_flux(const float4 WL, const Prim& L, const 
 __device__ __forceinline__ void diss_hllc(const float4 WL, const Prim& L, const float4 WR, const Prim& R, float nx,
                                           float ny, const Phys& ph, float D[4]) {
   float F[4], FL[4], FR[4];
-  hllc_flux(WL, L, WR, R, nx, ny, ph, F, FL, FR);
+  hllc_flux(WL, L, WR, R, nx, ny, ph, F);
+  phys_flux(WL, L, L.u * nx + L.v * ny, nx, ny, ph.invM2, FL);
+  phys_flux(WR, R, R.u * nx + R.v * ny, nx, ny, ph.invM2, FR);
 #pragma unroll
   for (int v = 0; v < 4; ++v) D[v] = 0.5f * (FL[v] + FR[v]) - F[v];
 }
@@ -676,8 +678,7 @@ __device__ __forceinline__ void face_flux(const float4 WL, const Prim& L, const 
   // HLLC and AUSM+ define their dissipation as central - F_scheme, so the
   // total flux central - D is the scheme's own flux (to rounding)
   if (!IR && FLUX == ESR_FLUX_HLLC) {
-    float FL[4], FR[4];
-    hllc_flux(WL, L, WR, R, nx, ny, ph, F, FL, FR);
+    hllc_flux(WL, L, WR, R, nx, ny, ph, F);
     return;
   }
   if (!IR && FLUX == ESR_FLUX_AUSM) {
@@ -726,6 +727,9 @@ __device__ __forceinline__ void face_flux(const float4 WL, const Prim& L, const 
 }
 
 // Conserved left (owner) and right (partner or ghost) states of a logical face.
+// Both states are loaded before either is used (a boundary face reads the
+// owner's element in place of a partner, in bounds, and replaces it by the
+// ghost state), so the partner load does not wait behind the face's branch.
 template <int RECON>
 __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int32_t c, const CaseDev& cs,
                                             const float4* __restrict__ Ws, const float4* __restrict__ rec,
@@ -738,29 +742,37 @@ __device__ __forceinline__ void face_states(const GroupDev* G, int32_t face, int
   const uint32_t f = ld(m.f_ref, (uint32_t)face);
   const int32_t i = (int32_t)(own & FREF_IDX);
   const int k = (int)(own >> 30);
+  const bool bnd = fref_is_bnd(f);
+  const int32_t j = bnd ? i : (int32_t)(f & FREF_IDX);
+  float4 qL, qR;
   if (RECON == ESR_RECON_MUSCL) {
-    L = as_prim(ld(rg, rel(G, k, i, c)));
-    if (fref_is_bnd(f)) {
-      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float4 Wi = ld(Wsg, sel(G, i, c));
-      const float2 nk = (kind == G_WALL) ? ld(m.f_nrm, (uint32_t)face) : make_float2(0.f, 0.f);
-      const float4 Wgh = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Wsg,
-                                     sel(G, (int32_t)(f & FREF_BIDX), c));
-      R = prim_of(Wgh, ph.M2, ph.gm1);
+    qL = ld(rg, rel(G, k, i, c));
+    qR = ld(rg, rel(G, bnd ? k : (int)(f >> 30), j, c));
+  } else {
+    qL = ld(Wsg, sel(G, i, c));
+    qR = ld(Wsg, sel(G, j, c));
+  }
+  const float4 value = cs.value;
+  if (bnd) {
+    const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
+    const float4 Wi = (RECON == ESR_RECON_MUSCL) ? ld(Wsg, sel(G, i, c)) : qL;
+    const float2 nk = (kind == G_WALL) ? ld(m.f_nrm, (uint32_t)face) : make_float2(0.f, 0.f);
+    const float4 Wgh = ghost_state(kind, Wi, nk, value, m.inlet_subsonic, Wsg, sel(G, (int32_t)(f & FREF_BIDX), c));
+    if (RECON == ESR_RECON_MUSCL) {
+      const Prim g = prim_of(Wgh, ph.M2, ph.gm1);
+      qR = make_float4(g.r, g.u, g.v, g.P);
     } else {
-      R = as_prim(ld(rg, rel(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), c)));
+      qR = Wgh;
     }
+  }
+  if (RECON == ESR_RECON_MUSCL) {
+    L = as_prim(qL);
+    R = as_prim(qR);
     WL = cons_of(L, ph.inv_gm1M2);
     WR = cons_of(R, ph.inv_gm1M2);
   } else {
-    WL = ld(Wsg, sel(G, i, c));
-    if (fref_is_bnd(f)) {
-      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nk = (kind == G_WALL) ? ld(m.f_nrm, (uint32_t)face) : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, WL, nk, cs.value, m.inlet_subsonic, Wsg, sel(G, (int32_t)(f & FREF_BIDX), c));
-    } else {
-      WR = ld(Wsg, sel(G, (int32_t)(f & FREF_IDX), c));
-    }
+    WL = qL;
+    WR = qR;
     L = prim_of(WL, ph.M2, ph.gm1);
     R = prim_of(WR, ph.M2, ph.gm1);
   }
@@ -796,11 +808,11 @@ __global__ __launch_bounds__(BLOCK) void k_face(const GroupDev* __restrict__ gro
   const GroupDev* G = it.G;
   const MeshDev& m = G->m;
   const CaseDev& cs = cases[G->case0 + it.c];
+  const float2 nk = ld(m.f_nrm, (uint32_t)it.idx);
+  const float sk = ld(m.f_len, (uint32_t)it.idx);
   float4 WL, WR;
   Prim L, R;
   face_states<RECON>(G, it.idx, it.c, cs, Ws, rec, WL, L, WR, R);
-  const float2 nk = ld(m.f_nrm, (uint32_t)it.idx);
-  const float sk = ld(m.f_len, (uint32_t)it.idx);
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   float F[4];
   class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, G->ph, lf_c, F);
@@ -838,32 +850,55 @@ __global__ __launch_bounds__(BLOCK) void k_lf_cmax(const GroupDev* __restrict__ 
 // are the group's base pointers)
 // LOCAL: local time stepping (each cell's own dt, CaseDev.cfl_local); a
 // compile-time choice, so the global-step kernels carry none of its code.
+// The inputs of the update (1/area, sponge weight, the step's start state W0,
+// the case's inlet state, time steps and sponge strength) are loaded by
+// stage_loads at the top of a kernel, with its other loads, not after the
+// fluxes.
+struct StageIn {
+  float ia, sg;
+  float4 w0;
+  float4 value;
+  float dt, dt_half, dt_sixth, strength;
+};
+__device__ __forceinline__ StageIn stage_loads(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell,
+                                               uint32_t s, const float4 Wi, const float4* W0) {
+  StageIn in;
+  in.ia = ld(G->m.inv_area, (uint32_t)cell);
+  in.sg = G->sponge ? ld(G->sponge_g, (uint32_t)cell) : 0.0f;
+  in.w0 = (mode == MODE_EE || mode == MODE_RESID) ? Wi : ld(W0, s);
+  in.value = cs.value;
+  in.dt = cs.dt;
+  in.dt_half = cs.dt_half;
+  in.dt_sixth = cs.dt_sixth;
+  in.strength = cs.strength;
+  return in;
+}
+
 template <bool LOCAL>
 __device__ __forceinline__ void stage_epilogue(int mode, const GroupDev* G, const CaseDev& cs, int32_t cell, uint32_t s,
-                                               const float4 Wi, const float4 Fk[3], const float4* W0, float4* out,
+                                               const float4 Wi, const float4 Fk[3], const StageIn& in, float4* out,
                                                float4* acc) {
-  const MeshDev& m = G->m;
-  const float ia = ld(m.inv_area, (uint32_t)cell);
+  const float ia = in.ia;
   float R[4];
   R[0] = ((Fk[0].x + Fk[1].x) + Fk[2].x) * ia;
   R[1] = ((Fk[0].y + Fk[1].y) + Fk[2].y) * ia;
   R[2] = ((Fk[0].z + Fk[1].z) + Fk[2].z) * ia;
   R[3] = ((Fk[0].w + Fk[1].w) + Fk[2].w) * ia;
   if (G->sponge) {
-    const float sigma = cs.strength * ld(G->sponge_g, (uint32_t)cell);
-    R[0] = R[0] - sigma * (cs.value.x - Wi.x);
-    R[1] = R[1] - sigma * (cs.value.y - Wi.y);
-    R[2] = R[2] - sigma * (cs.value.z - Wi.z);
-    R[3] = R[3] - sigma * (cs.value.w - Wi.w);
+    const float sigma = in.strength * in.sg;
+    R[0] = R[0] - sigma * (in.value.x - Wi.x);
+    R[1] = R[1] - sigma * (in.value.y - Wi.y);
+    R[2] = R[2] - sigma * (in.value.z - Wi.z);
+    R[3] = R[3] - sigma * (in.value.w - Wi.w);
   }
   if (mode == MODE_RESID) {
     st(out, s, make_float4(R[0], R[1], R[2], R[3]));
     return;
   }
-  const float4 w0 = (mode == MODE_EE) ? Wi : ld(W0, s);
+  const float4 w0 = in.w0;
   // time step: the case's global dt, or with local time stepping the cell's
   // own dt from the step's start state w0 (the same dt in every stage)
-  float dt = cs.dt, dt_half = cs.dt_half, dt_sixth = cs.dt_sixth;
+  float dt = in.dt, dt_half = in.dt_half, dt_sixth = in.dt_sixth;
   if (LOCAL) {
     dt = cell_dt1(G->m, G->ph, cell, w0) * cs.cfl_local;
     dt_half = dt / 2.0f;
@@ -909,6 +944,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
   const int32_t n = m.n, cell = it.idx;
   const CaseDev& cs = cases[G->case0 + it.c];
   const float4* fg = fx + G->fx_off;
+  const int64_t so = G->st_off;
+  const float4 Wi = ld(Ws + so, it.e);
+  const StageIn in = stage_loads(MODE, G, cs, cell, it.e, Wi, W0 + so);
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -917,8 +955,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const GroupDev* __restrict__ g
     Fk[k] = (cf & CFACE_NEG) ? make_float4(-f.x, -f.y, -f.z, -f.w) : f;
   }
   if (!it.active) return;
-  const int64_t so = G->st_off;
-  stage_epilogue<LOCAL>(MODE, G, cs, cell, it.e, ld(Ws + so, it.e), Fk, W0 + so, out + so, acc ? acc + so : acc);
+  stage_epilogue<LOCAL>(MODE, G, cs, cell, it.e, Wi, Fk, in, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -947,42 +984,74 @@ __global__ __launch_bounds__(BLOCK) void k_cell(const GroupDev* __restrict__ gro
   const float lf_c = (FLUX == ESR_FLUX_LF) ? __uint_as_float(cs.lf_cmax) : 0.0f;
   // constant reconstruction: the cell's own state on every face
   const Prim Pi = (RECON == ESR_RECON_MUSCL) ? Prim{} : prim_of(Wi, ph.M2, ph.gm1);
+  // Every load of the three faces is issued before any flux: the face
+  // records, then the cell's own face states and the partners' (a boundary
+  // face reads an in-bounds stand-in, its own face state or the cell's state,
+  // and replaces it by the ghost state below). Loaded face by face, each
+  // face's partner load would wait for the previous face's flux.
+  uint32_t f[3];
+  float2 nk[3];
+  float sk[3];
+  float4 qL[3], qR[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[k] = ld(m.fref, kn(k, n, cell));
+    nk[k] = ld(m.nrm, kn(k, n, cell));
+    sk[k] = ld(m.len, kn(k, n, cell));
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const bool bnd = fref_is_bnd(f[k]);
+    const int32_t j = bnd ? cell : (int32_t)(f[k] & FREF_IDX);
+    if (RECON == ESR_RECON_MUSCL) {
+      qL[k] = ld(rg, rel(G, k, cell, it.c));
+      qR[k] = ld(rg, rel(G, bnd ? k : (int)(f[k] >> 30), j, it.c));
+    } else {
+      qR[k] = ld(Wg, sel(G, j, it.c));
+    }
+  }
+  const int64_t so = G->st_off;
+  const StageIn in = stage_loads(mode, G, cs, cell, it.e, Wi, W0 + so);
   float4 Fk[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const uint32_t f = ld(m.fref, kn(k, n, cell));
-    const float2 nk = ld(m.nrm, kn(k, n, cell));
-    const float sk = ld(m.len, kn(k, n, cell));
     float4 WL, WR;
     Prim L, R;
     if (RECON == ESR_RECON_MUSCL) {
-      L = as_prim(ld(rg, rel(G, k, cell, it.c)));
+      L = as_prim(qL[k]);
       WL = cons_of(L, ph.inv_gm1M2);
     } else {
       L = Pi;
       WL = Wi;
     }
-    if (fref_is_bnd(f)) {
-      const uint32_t kind = (f >> FREF_KIND_SHIFT) & 7u;
-      const float2 nw = (kind == G_WALL) ? nk : make_float2(0.f, 0.f);
-      const float4 Wgh = ghost_state(kind, Wi, nw, cs.value, m.inlet_subsonic, Wg,
-                                     sel(G, (int32_t)(f & FREF_BIDX), it.c));
-      R = prim_of(Wgh, ph.M2, ph.gm1);
-      WR = (RECON == ESR_RECON_MUSCL) ? cons_of(R, ph.inv_gm1M2) : Wgh;
-    } else if (RECON == ESR_RECON_MUSCL) {
-      R = as_prim(ld(rg, rel(G, (int)(f >> 30), (int32_t)(f & FREF_IDX), it.c)));
+    // partner state: the loaded one, or for a boundary face the ghost state
+    // (in primitive form for MUSCL, whose both paths build WR from R)
+    float4 q = qR[k];
+    if (fref_is_bnd(f[k])) {
+      const uint32_t kind = (f[k] >> FREF_KIND_SHIFT) & 7u;
+      const float2 nw = (kind == G_WALL) ? nk[k] : make_float2(0.f, 0.f);
+      const float4 Wgh = ghost_state(kind, Wi, nw, in.value, m.inlet_subsonic, Wg,
+                                     sel(G, (int32_t)(f[k] & FREF_BIDX), it.c));
+      if (RECON == ESR_RECON_MUSCL) {
+        const Prim g = prim_of(Wgh, ph.M2, ph.gm1);
+        q = make_float4(g.r, g.u, g.v, g.P);
+      } else {
+        q = Wgh;
+      }
+    }
+    if (RECON == ESR_RECON_MUSCL) {
+      R = as_prim(q);
       WR = cons_of(R, ph.inv_gm1M2);
     } else {
-      WR = ld(Wg, sel(G, (int32_t)(f & FREF_IDX), it.c));
+      WR = q;
       R = prim_of(WR, ph.M2, ph.gm1);
     }
     float F[4];
-    class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk.x, nk.y, ph, lf_c, F);
-    Fk[k] = make_float4(sk * F[0], sk * F[1], sk * F[2], sk * F[3]);
+    class_flux<FLUX, ENT>(G->flux, WL, L, WR, R, nk[k].x, nk[k].y, ph, lf_c, F);
+    Fk[k] = make_float4(sk[k] * F[0], sk[k] * F[1], sk[k] * F[2], sk[k] * F[3]);
   }
   if (!it.active) return;
-  const int64_t so = G->st_off;
-  stage_epilogue<false>(mode, G, cs, cell, it.e, Wi, Fk, W0 + so, out + so, acc ? acc + so : acc);
+  stage_epilogue<false>(mode, G, cs, cell, it.e, Wi, Fk, in, out + so, acc ? acc + so : acc);
 }
 
 // ---------------------------------------------------------------------------
