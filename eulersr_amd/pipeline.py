@@ -376,7 +376,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
             # runs end on multiples of `chunk` (a multiple of the check
             # period), so replaced cases keep the check cadence of a solo run
             step = bs.run((step // chunk + 1) * chunk)
-            reps = bs.reports()
+            reps = bs.reports_lazy()
             for j, ci in enumerate(slot_hedge):
                 if ci is None:
                     continue

@@ -49,6 +49,25 @@ def get_stage_path() -> str:
     return {L.STAGE_CELL: "cell", L.STAGE_FACE: "face"}[L.lib().esr_get_stage_path()]
 
 
+class _LazyReports:
+    """Indexable reports of a solver: entry j becomes its dict on first read."""
+    __slots__ = ("_arr", "_conv", "_cache")
+
+    def __init__(self, arr, conv):
+        self._arr, self._conv, self._cache = arr, conv, {}
+
+    def __len__(self):
+        return len(self._arr)
+
+    def __getitem__(self, j):
+        d = self._cache.get(j)
+        if d is None:
+            if not 0 <= j < len(self._arr):
+                raise IndexError(j)
+            d = self._cache[j] = self._conv(self._arr[j])
+        return d
+
+
 class BatchSolver:
     def __init__(self, cases, scheme="SRK2", threshold=1e-6, check_every=5, patience=1, max_steps=0,
                  graph=True, autotune=False, launch=None, local_dt=False):
@@ -182,6 +201,14 @@ class BatchSolver:
         arr = (L.CaseReport * self.n)()
         L.check(L.lib().esr_solver_reports(self.handle, self.n, arr), "esr_solver_reports")
         return [self._report_dict(r) for r in arr]
+
+    def reports_lazy(self):
+        """Reports of every case from one device copy, each converted to its
+        dict when first indexed: a continuous-batching loop reads only the
+        slots in use, out of every slot of every sub-group."""
+        arr = (L.CaseReport * self.n)()
+        L.check(L.lib().esr_solver_reports(self.handle, self.n, arr), "esr_solver_reports")
+        return _LazyReports(arr, self._report_dict)
 
     @staticmethod
     def _report_dict(r) -> dict:

@@ -72,6 +72,9 @@ class MockSolver:
     def reports(self):
         return [self._report(j) for j in range(len(self.specs))]
 
+    def reports_lazy(self):
+        return self.reports()
+
     def replace_case(self, j, spec, W):
         assert P._kind(spec) == P._kind(self.specs[j]), "a slot only takes cases of its own kind"
         g = self.specs[j].group
@@ -284,3 +287,15 @@ def test_cases_already_at_their_fallback_are_neither_hedged_nor_restarted(mock):
     bs = mock.instances[0]
     assert not [x for x in bs.log if x[0] == "replace" and x[2] == 3 and x[3] > 1000]
     assert not reps[3]["converged"] and reps[3]["steps_total"] == reps[3]["steps_run"]
+
+
+def test_lazy_reports_convert_each_slot_once_on_demand():
+    import ctypes as C
+    from eulersr_amd.solve import _LazyReports
+    calls = []
+    arr = (C.c_int * 600)(*range(600))
+    reps = _LazyReports(arr, lambda x: calls.append(x) or {"v": x})
+    assert len(reps) == 600 and calls == []
+    assert reps[17] == {"v": 17} and reps[17] is reps[17] and calls == [17]
+    with pytest.raises(IndexError):
+        reps[600]
