@@ -22,6 +22,7 @@
 from __future__ import annotations
 
 import copy
+import math
 import time
 from dataclasses import dataclass
 
@@ -201,6 +202,41 @@ def stalled(hist, steps, after, window, factor):
     return bool(before) and bool(recent) and min(recent) > factor * min(before)
 
 
+class StallWatch:
+    """`stalled` for one solve whose check residuals arrive in step order,
+    kept incrementally: the checks at or before steps - window are a prefix
+    that only grows (its minimum kept as it grows), the later ones a window
+    whose minimum a monotonic queue holds, so each check costs O(1)
+    amortised instead of a pass over the whole history."""
+    __slots__ = ("hist", "p", "before_min", "q")
+
+    def __init__(self):
+        from collections import deque
+        self.hist = []                 # (steps, residual), steps non-decreasing
+        self.p = 0                     # hist[:p] lie at or before the window
+        self.before_min = math.inf
+        self.q = deque()               # indices >= p, residuals increasing: q[0] is the window's minimum
+
+    def add(self, steps, rel):
+        h, q = self.hist, self.q
+        h.append((steps, rel))
+        while q and h[q[-1]][1] >= rel:
+            q.pop()
+        q.append(len(h) - 1)
+
+    def stalled(self, steps, after, window, factor):
+        if steps < after:
+            return False
+        h, q = self.hist, self.q
+        cut = steps - window
+        while self.p < len(h) and h[self.p][0] <= cut:
+            self.before_min = min(self.before_min, h[self.p][1])
+            if q and q[0] == self.p:
+                q.popleft()
+            self.p += 1
+        return self.p > 0 and self.p < len(h) and h[q[0]][1] > factor * self.before_min
+
+
 def _queues(specs):
     from collections import OrderedDict, deque
     queues = OrderedDict()
@@ -293,6 +329,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     if chunk % CHECK_EVERY:
         raise ValueError(f"chunk must be a multiple of the check period {CHECK_EVERY}")
     specs = [copy.copy(sp) for sp in specs]
+    kinds = [_kind(sp) for sp in specs]
     if max_steps:
         for sp in specs:
             sp.max_steps = int(max_steps)
@@ -304,7 +341,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
     # the slots of sub-groups that start idle
     start = {}
     for ci in first:
-        start.setdefault(_kind(specs[ci]), []).append(ci)
+        start.setdefault(kinds[ci], []).append(ci)
     sub_kind, sub_slots, slot_case, slot_spec, slot_init = [], [], [], [], []
     tag = 0
     for k in list(queues.keys()):
@@ -348,9 +385,10 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
         bs.compact()
         if AUTOTUNE and LAUNCH is None:
             bs.tune()
-        hist = {ci: [] for ci in first}
+        hist = {ci: StallWatch() for ci in first}
         placed = {ci: step for ci in first}    # step at which each case entered its slot
         restarted = []
+        was_restarted = set()                  # the ids of `restarted`, for membership tests
         cold = set(cold_ids)
         spent = {}                             # case id -> steps of its discarded runs
         dropped = {}                           # case id -> last reports of its discarded warm attempts
@@ -390,6 +428,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     discard(ci, reps[p])
                     results[ci] = dict(r, steps_total=spent[ci] + r["steps_run"], discarded=dropped.get(ci, []))
                     restarted.append(ci)
+                    was_restarted.add(ci)
                     slot_hedge[j] = None
                     del hedge_of[ci]
                     changed = True
@@ -405,10 +444,10 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     continue
                 r = reps[j]
                 done = r["converged"] or r["diverged"] or r["steps_run"] >= r["N"]
-                if fallback_inits is not None and ci not in restarted and ci not in cold and not r["converged"]:
+                if fallback_inits is not None and ci not in was_restarted and ci not in cold and not r["converged"]:
                     if r["last_rel"] == r["last_rel"] and r["last_rel"] != float("inf"):
-                        hist[ci].append((r["steps_run"], r["last_rel"]))
-                    if r["diverged"] or (not done and stalled(hist[ci], r["steps_run"], *stall)):
+                        hist[ci].add(r["steps_run"], r["last_rel"])
+                    if r["diverged"] or (not done and hist[ci].stalled(r["steps_run"], *stall)):
                         spent[ci] = spent.get(ci, 0) + r["steps_run"]
                         discard(ci, r)
                         if ci in hedge_of:
@@ -422,6 +461,7 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                         else:
                             bs.replace_case(j, specs[ci], fallback_inits[ci])
                         restarted.append(ci)
+                        was_restarted.add(ci)
                         continue
                 if not done:
                     continue
@@ -431,12 +471,12 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     slot_hedge[h] = None
                     spent[ci] = spent.get(ci, 0) + reps[h]["steps_run"]
                 results[ci] = dict(r, steps_total=spent.get(ci, 0) + r["steps_run"], discarded=dropped.get(ci, []))
-                q = queues[_kind(specs[ci])]
+                q = queues[kinds[ci]]
                 if q:
                     nxt = q.popleft()
                     bs.replace_case(j, specs[nxt], inits[nxt])
                     slot_case[j] = nxt
-                    hist[nxt] = []
+                    hist[nxt] = StallWatch()
                     placed[nxt] = step
                 else:
                     slot_case[j] = None
@@ -444,11 +484,22 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
             # open idle sub-groups of the kinds with the most waiting cases
             # (a live sub-group costs all of its slots in every launch, so the
             # launched slots are bounded as well as the cases in flight)
-            active = sum(ci is not None for ci in slot_case)
-            lanes = S * sum(any(busy(j) for j in sl) for sl in sub_slots)
+            # (one pass over the slots: cases in flight, sub-groups in use)
+            active = 0
+            sub_busy = []
+            for sl in sub_slots:
+                b = False
+                for j in sl:
+                    if slot_case[j] is not None:
+                        active += 1
+                        b = True
+                    elif slot_hedge[j] is not None:
+                        b = True
+                sub_busy.append(b)
+            lanes = S * sum(sub_busy)
             while active <= slots - S and lanes + S <= LANE_FACTOR * slots:
                 waiting = [(len(queues[sub_kind[g]]), g) for g in range(len(sub_kind))
-                           if queues[sub_kind[g]] and not any(busy(j) for j in sub_slots[g])]
+                           if queues[sub_kind[g]] and not sub_busy[g]]
                 if not waiting:
                     break
                 g = max(waiting, key=lambda x: (x[0], -x[1]))[1]
@@ -459,9 +510,10 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                     nxt = q.popleft()
                     bs.replace_case(j, specs[nxt], inits[nxt])
                     slot_case[j] = nxt
-                    hist[nxt] = []
+                    hist[nxt] = StallWatch()
                     placed[nxt] = step
                     active += 1
+                sub_busy[g] = True
                 lanes += S
                 changed = True
             # hedges: free slots of launched sub-groups, once nothing waits
@@ -473,8 +525,8 @@ def campaign(specs, inits, slots=128, max_steps=0, chunk=100, fallback_inits=Non
                         if busy(j):
                             continue
                         cand = [(step - placed[ci], ci) for ci in slot_case
-                                if ci is not None and ci not in restarted and ci not in hedge_of
-                                and ci not in cold and _kind(specs[ci]) == sub_kind[g]]
+                                if ci is not None and ci not in was_restarted and ci not in hedge_of
+                                and ci not in cold and kinds[ci] == sub_kind[g]]
                         if not cand:
                             break
                         ci = max(cand)[1]

@@ -299,3 +299,18 @@ def test_lazy_reports_convert_each_slot_once_on_demand():
     assert reps[17] == {"v": 17} and reps[17] is reps[17] and calls == [17]
     with pytest.raises(IndexError):
         reps[600]
+
+
+def test_stall_watch_equals_the_stall_rule_on_random_histories():
+    import random
+    rng = random.Random(5)
+    for trial in range(300):
+        w, hist, steps = P.StallWatch(), [], 0
+        after, window = rng.choice([(0, 300), (6000, 3000), (1000, 500)])
+        for _ in range(rng.randint(0, 120)):
+            steps += rng.choice([5, 100, 100, 200])
+            # residuals falling, flat or noisy, with repeats
+            r = rng.choice([1e-3 / (1 + steps), 1e-4, rng.uniform(1e-6, 1e-3), 3e-5])
+            hist.append((steps, r))
+            w.add(steps, r)
+            assert w.stalled(steps, after, window, 0.97) == P.stalled(hist, steps, after, window, 0.97)
