@@ -280,9 +280,10 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
 
   // Every load is issued before the arithmetic: the face records, the
   // neighbours' states (a boundary face reads the cell's own state, in
-  // bounds, and replaces it by the ghost state below), the LSQ weights and
-  // the case's inlet state. Loaded face by face, each neighbour load would
-  // wait behind the previous face's branch.
+  // bounds, and replaces it by the ghost state below) and the LSQ weights.
+  // Loaded face by face, each neighbour load would wait behind the previous
+  // face's branch. (The case's inlet state, which only inlet ghosts read,
+  // stays in their branch: loaded for every cell it cost 4 % at 512 cases.)
   const float4 Wi = ld(Wg, it.e);
   uint32_t f[3];
 #pragma unroll
@@ -296,8 +297,6 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
   float2 dm[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) dm[k] = ld(m.dxm, kn(k, n, cell));
-  const float4 value = cs.value;
-
   const Prim pi = prim_of(Wi, ph.M2, ph.gm1);
   Prim pr[3];
   bool bnd[3];
@@ -308,7 +307,7 @@ __global__ __launch_bounds__(BLOCK) void k_recon(const GroupDev* __restrict__ gr
     if (bnd[k]) {
       const uint32_t kind = (f[k] >> FREF_KIND_SHIFT) & 7u;
       const float2 nk = (kind == G_WALL) ? ld(m.nrm, kn(k, n, cell)) : make_float2(0.f, 0.f);
-      WR = ghost_state(kind, Wi, nk, value, m.inlet_subsonic, Wg, sel(G, (int32_t)(f[k] & FREF_BIDX), it.c));
+      WR = ghost_state(kind, Wi, nk, cs.value, m.inlet_subsonic, Wg, sel(G, (int32_t)(f[k] & FREF_BIDX), it.c));
     }
     pr[k] = prim_of(WR, ph.M2, ph.gm1);
   }
