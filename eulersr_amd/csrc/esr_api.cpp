@@ -988,6 +988,13 @@ esr_status esr_solver_finalize(esr_solver* s, void* stream) {
     if (!s->prepared) throw Error(ESR_ERR_STATE, "esr_solver_prepare must be called first");
     hipStream_t st = S(stream);
     Engine& e = s->eng;
+    // the probe and the forces cover every case: groups that
+    // esr_solver_compact dropped from the launches are launched again
+    if (s->gs) hip_check(hipStreamSynchronize(s->gs), "solver stream sync");
+    if (!e.group_on.empty() && e.set_group_mask(std::vector<uint8_t>(e.groups.size(), 1), st) && s->chunk) {
+      (void)hipGraphExecDestroy(s->chunk);
+      s->chunk = nullptr;
+    }
     // probe: one more step from the final state (main.py:228-230)
     launch_copy(e.W0.as<float4>(), e.Wref.as<float4>(), e.elems, st);
     launch_set_active(e.n_cases, e.cases(), nullptr, 1, st);

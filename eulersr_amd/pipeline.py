@@ -66,16 +66,60 @@ def make_cases(mesh, geometry, grid, **over):
     return specs, inits
 
 
-def fixed_time(specs, inits, max_steps=0):
+PACK_SIZE, PACK_CHUNK = 16, 250   # fixed_time(pack=True): cases per sub-group, steps between compactions
+
+
+def _steps_key(sp):
+    """tf x (|u| + c) / CFL of a case's inlet state: on one mesh a freestream
+    start's dt is CFL x (a mesh quantity) / (|u| + c), so this orders its N."""
+    kw = sp.kw
+    g, m = float(kw.get("gamma", 1.4)), float(kw.get("M", 1.0))
+    r, ru, rv, E = (float(x) for x in kw["value"])
+    u, v = ru / r, rv / r
+    P = (E - 0.5 * r * (u * u + v * v)) * m * m * (g - 1.0)
+    c = math.sqrt(abs(g * P / r)) / m
+    return sp.tf * (math.hypot(u, v) + c) / sp.cfl
+
+
+def fixed_time(specs, inits, max_steps=0, pack=False):
     """Runs every case to its N = int(tf/dt)+1 (no early stop). Returns the
-    reports (probe residual, CD, CL) and the final states (CUDA tensors)."""
+    reports (probe residual, CD, CL) and the final states (CUDA tensors).
+
+    pack: the cases of each kind run in launch sub-groups of PACK_SIZE taken
+    in the order of their N (`_steps_key`), and every PACK_CHUNK steps the
+    sub-groups whose cases are all done drop out of the launches
+    (esr_solver_compact); otherwise every case is computed (its stores
+    skipped once done) until the last one ends. A case's state does not
+    depend on its group; the FP64 residuals of the reports do at the 1e-12
+    level (their partial sums are laid out by the group's case count), so
+    solves whose residuals set thresholds keep pack off."""
+    if pack:
+        specs = [copy.copy(sp) for sp in specs]
+        by_kind = {}
+        for i, sp in enumerate(specs):
+            by_kind.setdefault(_kind(sp), []).append(i)
+        tag = 0
+        for ids in by_kind.values():
+            ids.sort(key=lambda i: _steps_key(specs[i]))
+            for j in range(0, len(ids), PACK_SIZE):
+                tag += 1
+                for i in ids[j:j + PACK_SIZE]:
+                    specs[i].group = tag
     bs = BatchSolver(specs, scheme="SRK2", threshold=0.0, check_every=CHECK_EVERY, patience=PATIENCE,
                      max_steps=max_steps, autotune=AUTOTUNE, launch=LAUNCH)
     try:
         for i, W in enumerate(inits):
             bs.set_state(i, W)
         bs.prepare()
-        bs.run()
+        if pack:
+            step = 0
+            while True:
+                nxt = bs.run(step + PACK_CHUNK)
+                if bs.compact() == 0 or nxt == step:
+                    break
+                step = nxt
+        else:
+            bs.run()
         bs.finalize()
         reps = [bs.report(i) for i in range(len(specs))]
         states = [bs.get_state(i) for i in range(len(specs))]
@@ -140,7 +184,7 @@ def training_pairs(geometry, lr_mesh, hr_mesh, grid, max_steps=0):
     import torch
     ls, li = make_cases(lr_mesh, geometry, grid)
     hs, hi = make_cases(hr_mesh, geometry, grid)
-    reps, states = fixed_time(ls + hs, li + hi, max_steps=max_steps)
+    reps, states = fixed_time(ls + hs, li + hi, max_steps=max_steps, pack=True)
     n = len(grid)
     return dict(lr=torch.stack(states[:n]), hr=torch.stack(states[n:]), lr_reports=reps[:n], hr_reports=reps[n:],
                 machs=np.array([g[0] for g in grid]), aoas=np.array([g[1] for g in grid]))
@@ -605,7 +649,8 @@ class WarmStarter:
         if mode != "cold":
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps)
+            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps,
+                                      pack=True)
             torch.cuda.synchronize()
             t_lr = time.perf_counter() - t0
             t0 = time.perf_counter()
@@ -656,7 +701,8 @@ class WarmStarter:
         inits = [c.hr_init for c in cases]
         if mode != "cold":
             t0 = time.perf_counter()
-            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps)
+            _, lr_states = fixed_time([c.lr_spec for c in cases], [c.lr_init for c in cases], max_steps=max_steps,
+                                      pack=True)
             torch.cuda.synchronize()
             out["lr_s"] = time.perf_counter() - t0
             t0 = time.perf_counter()

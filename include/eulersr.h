@@ -233,7 +233,8 @@ esr_status esr_solver_replace_case(esr_solver* s, int32_t c, const esr_case* spe
  * null) receives the number of groups still launched. No counterpart in the
  * reference. */
 esr_status esr_solver_compact(esr_solver* s, int32_t* groups_on, void* stream);
-/* One-step probe residual + CD/CL of the current states. */
+/* One-step probe residual + CD/CL of the current states, for every case
+ * (groups dropped by esr_solver_compact are launched again). */
 esr_status esr_solver_finalize(esr_solver* s, void* stream);
 esr_status esr_solver_report(const esr_solver* s, int32_t c, esr_case_report* out);
 /* Reports of all n cases (n = the solver's case count) with one device copy. */

@@ -314,3 +314,23 @@ def test_stall_watch_equals_the_stall_rule_on_random_histories():
             hist.append((steps, r))
             w.add(steps, r)
             assert w.stalled(steps, after, window, 0.97) == P.stalled(hist, steps, after, window, 0.97)
+
+
+def test_steps_key_orders_cases_by_their_step_count():
+    """On one mesh, fixed_time's packing key orders freestream starts as
+    their N = int(tf / dt) + 1 does, with dt from the oracle's get_dt."""
+    import numpy as np
+    from conftest import base_cfg, mesh_of
+    from eulersr_amd import main as M
+    from oracle import fvm as F
+    mesh = mesh_of("naca0012", 0.2)
+    om = F.omesh(mesh, np.float64)
+    keys, ns = [], []
+    for mach, aoa in ((0.7, 0.0), (0.85, 3.0), (1.05, -2.0), (1.2, 5.0), (1.8, 1.0), (2.4, -4.0), (3.0, 2.0)):
+        cfg = base_cfg(Mach=mach, aoa=aoa, case="naca0012", tf=1.0)
+        W, inlet = M.initialize(mesh, cfg)
+        sp = M.case_spec(mesh, inlet, cfg)
+        dt = F.get_dt(np.asarray(W, np.float64), om, CFL=sp.cfl)
+        keys.append(P._steps_key(sp))
+        ns.append(int(sp.tf / dt) + 1)
+    assert np.argsort(keys).tolist() == np.argsort(ns).tolist(), (keys, ns)
